@@ -1,0 +1,13 @@
+"""dpgsolver_amd: MI355X-native explicit DG residual path (volume + face residual, inverse mass,
+SSP-RK3 stage with positivity) of DPGSolver, behind the C-ABI in include/dgx.h.
+
+The compute path is libdgx.so (hand-written gfx950 kernels); this package only holds ctypes bindings,
+the host-side set-up helpers and the solver mirror.
+"""
+from . import _ffi
+from .problem import (Geometry, Mesh, dt_cfl, geometry, ic_isentropic_vortex, ic_taylor_green, randn,
+                      solution_nodes_xyz, structured_mesh, tgv_3d, vortex_2d)
+from .solver import Solver
+
+__all__ = ["Mesh", "Geometry", "Solver", "structured_mesh", "geometry", "ic_isentropic_vortex",
+           "ic_taylor_green", "dt_cfl", "randn", "solution_nodes_xyz", "vortex_2d", "tgv_3d"]

@@ -1,0 +1,523 @@
+// C-ABI of include/dgx.h: context, uploads, and the stage/residual entry points.
+// Host code only; every device computation is a kernel in dgx_stage.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dgx.h"
+#include "dgx_setup.h"
+#include "dgx_internal.hpp"
+
+struct dgx_ctx {
+    int d = 0, p = 0, nv = 0, ns = 0, nf = 0, nface = 0;
+    int flux = 0, positivity = 1, precision = 0;
+    int64_t E = 0, F = 0;
+    int64_t grid = 0;
+    hipStream_t stream = nullptr;
+    double* u = nullptr;         // [nv][E][ns]
+    double* up = nullptr;        // [nv][E][ns]
+    double* rhs = nullptr;       // [nv][E][ns] residual / staging buffer
+    double* met = nullptr;       // [d*d][E][ns]
+    double* imass = nullptr;     // [E][ns]
+    double* tr[2] = { nullptr, nullptr };   // [E][nface][nv][nf]
+    uint32_t* nbr = nullptr;     // [E][nface]
+    double* blockmax = nullptr;  // [grid]
+    double* dmax = nullptr;      // [1]
+    unsigned long long* bad_elem = nullptr;
+    unsigned int* nonfinite = nullptr;
+    int cur = 0;                 // index of the trace buffer holding the current state's traces
+    hipGraphExec_t graph[2] = { nullptr, nullptr };
+    double graph_dt = 0.0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    std::string err;
+};
+
+namespace {
+
+dgx_status fail(dgx_ctx* c, dgx_status s, const std::string& msg)
+{
+    if (c)
+        c->err = msg;
+    return s;
+}
+
+#define HIP_TRY(ctx, call)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail((ctx), DGX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+void free_all(dgx_ctx* c)
+{
+    for (int i = 0; i < 2; ++i) {
+        if (c->graph[i])
+            (void)hipGraphExecDestroy(c->graph[i]);
+        c->graph[i] = nullptr;
+    }
+    void* ptrs[] = { c->u, c->up, c->rhs, c->met, c->imass, c->tr[0], c->tr[1], c->nbr, c->blockmax,
+                     c->dmax, c->bad_elem, c->nonfinite };
+    for (void* p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    if (c->ev0)
+        (void)hipEventDestroy(c->ev0);
+    if (c->ev1)
+        (void)hipEventDestroy(c->ev1);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+}
+
+dgx_status reset_flags(dgx_ctx* c)
+{
+    HIP_TRY(c, dgx::launch_fill_u64(c->bad_elem, 1, ~0ull, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->nonfinite, 0, sizeof(unsigned int), c->stream));
+    return DGX_OK;
+}
+
+dgx_status check_flags(dgx_ctx* c, int64_t* elem)
+{
+    unsigned long long bad = 0;
+    unsigned int nf = 0;
+    HIP_TRY(c, hipMemcpyAsync(&bad, c->bad_elem, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(&nf, c->nonfinite, sizeof(nf), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (elem)
+        *elem = (bad == ~0ull) ? -1 : (int64_t)bad;
+    if (bad != ~0ull)
+        return fail(c, DGX_ERR_POSITIVITY,
+                    "average density or pressure below EPS_PHYS in element " + std::to_string(bad));
+    if (nf)
+        return fail(c, DGX_ERR_NONFINITE, "non-finite residual");
+    return DGX_OK;
+}
+
+dgx::StageArgs base_args(dgx_ctx* c)
+{
+    dgx::StageArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.u = c->u;
+    a.up = c->up;
+    a.met = c->met;
+    a.imass = c->imass;
+    a.tr_in = c->tr[c->cur];
+    a.tr_out = c->tr[c->cur ^ 1];
+    a.nbr = c->nbr;
+    a.rhs_out = c->rhs;
+    a.blockmax = c->blockmax;
+    a.bad_elem = c->bad_elem;
+    a.n_elem = c->E;
+    a.positivity = c->positivity;
+    return a;
+}
+
+dgx_status read_max(dgx_ctx* c, double* max_rhs)
+{
+    HIP_TRY(c, dgx::launch_reduce_max(c->blockmax, c->grid, c->dmax, c->nonfinite, c->stream));
+    if (max_rhs)
+        HIP_TRY(c, hipMemcpyAsync(max_rhs, c->dmax, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return DGX_OK;
+}
+
+// Converts between a caller buffer and the device SoA state buffer dst/src.
+dgx_status upload_state(dgx_ctx* c, int layout, int ptr_kind, const double* src, double* dst_soa)
+{
+    const size_t bytes = sizeof(double) * (size_t)c->E * c->nv * c->ns;
+    if (layout == DGX_LAYOUT_SOA) {
+        HIP_TRY(c, hipMemcpyAsync(dst_soa, src, bytes, ptr_kind == DGX_PTR_HOST ? hipMemcpyHostToDevice
+                                                                                 : hipMemcpyDeviceToDevice,
+                                  c->stream));
+        return DGX_OK;
+    }
+    const double* ref = src;
+    if (ptr_kind == DGX_PTR_HOST) {
+        double* stage = (dst_soa == c->rhs) ? c->up : c->rhs;
+        HIP_TRY(c, hipMemcpyAsync(stage, src, bytes, hipMemcpyHostToDevice, c->stream));
+        ref = stage;
+    }
+    HIP_TRY(c, dgx::launch_ref_to_soa(ref, dst_soa, c->E, c->nv, c->ns, c->stream));
+    return DGX_OK;
+}
+
+dgx_status download_state(dgx_ctx* c, int layout, int ptr_kind, const double* src_soa, double* dst, double* scratch)
+{
+    const size_t bytes = sizeof(double) * (size_t)c->E * c->nv * c->ns;
+    if (layout == DGX_LAYOUT_SOA) {
+        HIP_TRY(c, hipMemcpyAsync(dst, src_soa, bytes, ptr_kind == DGX_PTR_HOST ? hipMemcpyDeviceToHost
+                                                                                 : hipMemcpyDeviceToDevice,
+                                  c->stream));
+        return DGX_OK;
+    }
+    if (ptr_kind == DGX_PTR_DEVICE) {
+        HIP_TRY(c, dgx::launch_soa_to_ref(src_soa, dst, c->E, c->nv, c->ns, c->stream));
+        return DGX_OK;
+    }
+    HIP_TRY(c, dgx::launch_soa_to_ref(src_soa, scratch, c->E, c->nv, c->ns, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(dst, scratch, bytes, hipMemcpyDeviceToHost, c->stream));
+    return DGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dgx_status_string(dgx_status s)
+{
+    switch (s) {
+    case DGX_OK: return "DGX_OK";
+    case DGX_ERR_INVALID_ARG: return "DGX_ERR_INVALID_ARG";
+    case DGX_ERR_HIP: return "DGX_ERR_HIP";
+    case DGX_ERR_POSITIVITY: return "DGX_ERR_POSITIVITY";
+    case DGX_ERR_NONFINITE: return "DGX_ERR_NONFINITE";
+    case DGX_ERR_GEOMETRY: return "DGX_ERR_GEOMETRY";
+    case DGX_ERR_NO_DEVICE: return "DGX_ERR_NO_DEVICE";
+    }
+    return "unknown";
+}
+
+const char* dgx_last_error(const dgx_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geometry* geom, dgx_ctx** out)
+{
+    if (!out)
+        return DGX_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!cfg || !mesh || !geom)
+        return DGX_ERR_INVALID_ARG;
+    const int d = cfg->d, p = cfg->p;
+    if (!dgx::supported_order(d, p) || mesh->d != d || cfg->pde != DGX_PDE_EULER ||
+        (cfg->num_flux != DGX_FLUX_LAX_FRIEDRICHS && cfg->num_flux != DGX_FLUX_ROE_PIKE) ||
+        (cfg->precision != DGX_PREC_FP64 && cfg->precision != DGX_PREC_MIXED))
+        return DGX_ERR_INVALID_ARG;
+    if (mesh->n_elem < 1 || mesh->n_elem >= (1 << 28) || mesh->n_face < 1 || !mesh->face_elem || !mesh->face_lf ||
+        !mesh->face_ord || !geom->metrics_vc || !geom->jdet_vc)
+        return DGX_ERR_INVALID_ARG;
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return DGX_ERR_NO_DEVICE;
+
+    dgx_ctx* c = new dgx_ctx();
+    c->d = d;
+    c->p = p;
+    c->nv = d + 2;
+    const int n1 = p + 1;
+    c->ns = (d == 2) ? n1 * n1 : n1 * n1 * n1;
+    c->nf = (d == 2) ? n1 : n1 * n1;
+    c->nface = 2 * d;
+    c->flux = cfg->num_flux;
+    c->positivity = cfg->positivity ? 1 : 0;
+    c->precision = cfg->precision;
+    c->E = mesh->n_elem;
+    c->F = mesh->n_face;
+    c->grid = dgx::stage_grid(d, p, c->E);
+    const int64_t E = c->E, ns = c->ns;
+
+    // neighbour table: every (element, local face) slot must be filled exactly once
+    std::vector<uint32_t> nbr((size_t)E * c->nface, 0xFFFFFFFFu);
+    for (int64_t f = 0; f < c->F; ++f) {
+        const int64_t eL = mesh->face_elem[2 * f], eR = mesh->face_elem[2 * f + 1];
+        const int lL = mesh->face_lf[2 * f], lR = mesh->face_lf[2 * f + 1];
+        if (eL < 0 || eL >= E || eR < 0 || eR >= E || lL < 0 || lL >= c->nface || lR < 0 || lR >= c->nface) {
+            delete c;
+            return DGX_ERR_INVALID_ARG;
+        }
+        if (mesh->face_ord[f] != 0) {
+            delete c;   // only aligned node correspondence is supported
+            return DGX_ERR_INVALID_ARG;
+        }
+        uint32_t& sl = nbr[eL * c->nface + lL];
+        uint32_t& sr = nbr[eR * c->nface + lR];
+        if (sl != 0xFFFFFFFFu || sr != 0xFFFFFFFFu) {
+            delete c;
+            return DGX_ERR_INVALID_ARG;
+        }
+        sl = dgx::pack_nbr((uint32_t)eR, (uint32_t)lR, 0u);
+        sr = dgx::pack_nbr((uint32_t)eL, (uint32_t)lL, 1u);
+    }
+    for (uint32_t v : nbr)
+        if (v == 0xFFFFFFFFu) {
+            delete c;   // boundary faces are not part of the periodic explicit path
+            return DGX_ERR_INVALID_ARG;
+        }
+
+    dgx_ops1d o;
+    if (dgx_ops1d_build(p, &o) != DGX_OK) {
+        delete c;
+        return DGX_ERR_INVALID_ARG;
+    }
+    dgx::Ops1D ops;
+    std::memset(&ops, 0, sizeof(ops));
+    for (int i = 0; i < n1; ++i) {
+        ops.w[i] = o.w[i];
+        ops.l[0][i] = o.lm[i];
+        ops.l[1][i] = o.lp[i];
+    }
+    for (int i = 0; i < n1 * n1; ++i) {
+        ops.D[i] = o.D[i];
+        ops.sb[i] = o.sb[i];
+        ops.bs[i] = o.bs[i];
+    }
+
+    // metrics -> [d*d][E][ns]; inverse of the diagonal mass 1/(w J)
+    std::vector<double> met((size_t)d * d * E * ns), imass((size_t)E * ns);
+    for (int64_t e = 0; e < E; ++e) {
+        for (int cc = 0; cc < d * d; ++cc)
+            std::memcpy(&met[((size_t)cc * E + e) * ns], geom->metrics_vc + ((size_t)e * d * d + cc) * ns,
+                        sizeof(double) * ns);
+        for (int n = 0; n < ns; ++n) {
+            const int i0 = n % n1, i1 = (n / n1) % n1, i2 = n / (n1 * n1);
+            double W = o.w[i0] * o.w[i1];
+            if (d == 3)
+                W *= o.w[i2];
+            const double J = geom->jdet_vc[e * ns + n];
+            if (!(J > 0.0)) {
+                delete c;
+                return DGX_ERR_GEOMETRY;
+            }
+            imass[e * ns + n] = 1.0 / (W * J);
+        }
+    }
+
+    const size_t sbytes = sizeof(double) * (size_t)E * c->nv * ns;
+    const size_t tbytes = sizeof(double) * (size_t)E * c->nface * c->nv * c->nf;
+    dgx_status st = DGX_OK;
+    auto step = [&](hipError_t e_, const char* what) {
+        if (st == DGX_OK && e_ != hipSuccess)
+            st = fail(c, DGX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_));
+    };
+    step(hipStreamCreate(&c->stream), "hipStreamCreate");
+    step(hipEventCreate(&c->ev0), "hipEventCreate");
+    step(hipEventCreate(&c->ev1), "hipEventCreate");
+    step(hipMalloc(&c->u, sbytes), "hipMalloc(u)");
+    step(hipMalloc(&c->up, sbytes), "hipMalloc(up)");
+    step(hipMalloc(&c->rhs, sbytes), "hipMalloc(rhs)");
+    step(hipMalloc(&c->met, sizeof(double) * met.size()), "hipMalloc(met)");
+    step(hipMalloc(&c->imass, sizeof(double) * imass.size()), "hipMalloc(imass)");
+    step(hipMalloc(&c->tr[0], tbytes), "hipMalloc(tr0)");
+    step(hipMalloc(&c->tr[1], tbytes), "hipMalloc(tr1)");
+    step(hipMalloc(&c->nbr, sizeof(uint32_t) * nbr.size()), "hipMalloc(nbr)");
+    step(hipMalloc(&c->blockmax, sizeof(double) * c->grid), "hipMalloc(blockmax)");
+    step(hipMalloc(&c->dmax, sizeof(double)), "hipMalloc(dmax)");
+    step(hipMalloc(&c->bad_elem, sizeof(unsigned long long)), "hipMalloc(bad)");
+    step(hipMalloc(&c->nonfinite, sizeof(unsigned int)), "hipMalloc(nonfinite)");
+    if (st == DGX_OK) {
+        step(hipMemcpy(c->met, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice), "copy met");
+        step(hipMemcpy(c->imass, imass.data(), sizeof(double) * imass.size(), hipMemcpyHostToDevice), "copy imass");
+        step(hipMemcpy(c->nbr, nbr.data(), sizeof(uint32_t) * nbr.size(), hipMemcpyHostToDevice), "copy nbr");
+        step(hipMemset(c->u, 0, sbytes), "memset u");
+        step(hipMemset(c->up, 0, sbytes), "memset up");
+        step(hipMemset(c->tr[0], 0, tbytes), "memset tr0");
+        step(hipMemset(c->tr[1], 0, tbytes), "memset tr1");
+        step(hipMemset(c->blockmax, 0, sizeof(double) * c->grid), "memset blockmax");
+        step(dgx::upload_ops(p, ops), "upload ops");
+    }
+    if (st == DGX_OK)
+        st = reset_flags(c);
+    if (st == DGX_OK) {
+        step(hipStreamSynchronize(c->stream), "sync");
+    }
+    if (st != DGX_OK) {
+        fprintf(stderr, "dgx_create: %s\n", c->err.c_str());
+        free_all(c);
+        delete c;
+        return st;
+    }
+    *out = c;
+    return DGX_OK;
+}
+
+dgx_status dgx_destroy(dgx_ctx* ctx)
+{
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    if (ctx->stream)
+        (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+    return DGX_OK;
+}
+
+void* dgx_get_stream(dgx_ctx* ctx)
+{
+    return ctx ? (void*)ctx->stream : nullptr;
+}
+
+int64_t dgx_state_size(const dgx_ctx* ctx)
+{
+    return ctx ? ctx->E * ctx->nv * ctx->ns : 0;
+}
+
+dgx_status dgx_set_state(dgx_ctx* ctx, int layout, int ptr_kind, const double* u)
+{
+    if (!ctx || !u || (layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
+        (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_set_state: bad arguments");
+    dgx_status st = upload_state(ctx, layout, ptr_kind, u, ctx->u);
+    if (st != DGX_OK)
+        return st;
+    ctx->cur = 0;
+    HIP_TRY(ctx, dgx::launch_trace(ctx->d, ctx->p, ctx->u, ctx->tr[0], ctx->E, ctx->stream));
+    st = reset_flags(ctx);
+    if (st != DGX_OK)
+        return st;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return DGX_OK;
+}
+
+dgx_status dgx_get_state(dgx_ctx* ctx, int layout, int ptr_kind, double* u)
+{
+    if (!ctx || !u || (layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
+        (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_get_state: bad arguments");
+    dgx_status st = download_state(ctx, layout, ptr_kind, ctx->u, u, ctx->rhs);
+    if (st != DGX_OK)
+        return st;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return DGX_OK;
+}
+
+dgx_status dgx_rhs(dgx_ctx* ctx, int precision, int apply_minv, int layout, int ptr_kind, double* rhs,
+                   double* max_rhs)
+{
+    if (!ctx || (precision != DGX_PREC_FP64 && precision != DGX_PREC_MIXED))
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: bad arguments");
+    if (rhs && ((layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
+                (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE)))
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: bad layout");
+    dgx::StageArgs a = base_args(ctx);
+    a.stage = -1;
+    a.apply_minv = apply_minv ? 1 : 0;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, dgx::launch_stage(ctx->d, ctx->p, ctx->flux, a, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    dgx_status st = read_max(ctx, max_rhs);
+    if (st != DGX_OK)
+        return st;
+    if (rhs) {
+        st = download_state(ctx, layout, ptr_kind, ctx->rhs, rhs, ctx->up);
+        if (st != DGX_OK)
+            return st;
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
+        ctx->last_ms = ms;
+    return check_flags(ctx, nullptr);
+}
+
+dgx_status dgx_rk_stage(dgx_ctx* ctx, int stage, double dt, double* max_rhs)
+{
+    if (!ctx || stage < 0 || stage > 2 || !std::isfinite(dt))
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rk_stage: bad arguments");
+    dgx::StageArgs a = base_args(ctx);
+    a.stage = stage;
+    a.dt = dt;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, dgx::launch_stage(ctx->d, ctx->p, ctx->flux, a, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->cur ^= 1;
+    dgx_status st = read_max(ctx, max_rhs);
+    if (st != DGX_OK)
+        return st;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
+        ctx->last_ms = ms;
+    return check_flags(ctx, nullptr);
+}
+
+static dgx_status build_graph(dgx_ctx* c, int parity, double dt)
+{
+    if (c->graph[parity]) {
+        (void)hipGraphExecDestroy(c->graph[parity]);
+        c->graph[parity] = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int cur = parity;
+    hipError_t le = hipSuccess;
+    for (int s = 0; s < 3 && le == hipSuccess; ++s) {
+        dgx::StageArgs a = base_args(c);
+        a.tr_in = c->tr[cur];
+        a.tr_out = c->tr[cur ^ 1];
+        a.stage = s;
+        a.dt = dt;
+        le = dgx::launch_stage(c->d, c->p, c->flux, a, c->stream);
+        cur ^= 1;
+    }
+    hipError_t ee = hipStreamEndCapture(c->stream, &g);
+    if (le != hipSuccess)
+        return fail(c, DGX_ERR_HIP, std::string("capture launch: ") + hipGetErrorString(le));
+    if (ee != hipSuccess)
+        return fail(c, DGX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+    hipError_t ie = hipGraphInstantiate(&c->graph[parity], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess)
+        return fail(c, DGX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    return DGX_OK;
+}
+
+dgx_status dgx_step(dgx_ctx* ctx, double dt, int nsteps, double* max_rhs)
+{
+    if (!ctx || nsteps < 0 || !std::isfinite(dt))
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_step: bad arguments");
+    if (nsteps == 0)
+        return DGX_OK;
+    if (dt != ctx->graph_dt || !ctx->graph[0] || !ctx->graph[1]) {
+        for (int par = 0; par < 2; ++par) {
+            dgx_status st = build_graph(ctx, par, dt);
+            if (st != DGX_OK)
+                return st;
+        }
+        ctx->graph_dt = dt;
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        HIP_TRY(ctx, hipGraphLaunch(ctx->graph[ctx->cur], ctx->stream));
+        ctx->cur ^= 1;   // three stages flip the trace parity
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    dgx_status st = read_max(ctx, max_rhs);
+    if (st != DGX_OK)
+        return st;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
+        ctx->last_ms = ms;
+    return check_flags(ctx, nullptr);
+}
+
+dgx_status dgx_sync(dgx_ctx* ctx, int64_t* elem)
+{
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    return check_flags(ctx, elem);
+}
+
+dgx_status dgx_vjp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* v, double* out)
+{
+    (void)apply_minv; (void)layout; (void)ptr_kind; (void)v; (void)out;
+    return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_vjp: not available yet");
+}
+
+dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* w, double* out)
+{
+    (void)apply_minv; (void)layout; (void)ptr_kind; (void)w; (void)out;
+    return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_jvp: not available yet");
+}
+
+double dgx_last_elapsed_ms(const dgx_ctx* ctx)
+{
+    return ctx ? ctx->last_ms : 0.0;
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+// Device kernels of the explicit DG residual path for gfx950 (CDNA4), FP64 plain form.
+//
+// One fused kernel evaluates a stage: it reads an element's state and metrics, its own face traces
+// are recomputed from the state in LDS, the neighbours' face traces come from a trace buffer the
+// previous stage wrote, the numerical flux is evaluated at every face point of the element (both
+// sides of a face evaluate the same flux with the left side's convention), then the weak divergence
+// (sum-factorized: one 1-D derivative contraction per direction), the face lift, the diagonal inverse
+// mass, the SSP-RK3 combination and the Bezier positivity limiter are applied, and the new state and
+// its face traces are written.  The state is read once and written once per stage.
+//
+// Reference path (file:line in DPGSolver):
+//   compute_rhs                 src/simulation/solvers/solve.c:133-149
+//   volume rlhs                 solvers/compute_volume_rlhs_T.c:145-177, compute_rlhs_T.c:71-110
+//   face rlhs                   solvers/compute_face_rlhs_T.c:51-146, 297-359; test_case/boundary/boundary_T.c:42-92
+//   Euler flux                  test_case/flux/flux_euler_T.c:71-143, 259-300
+//   LF / Roe-Pike               solvers/numerical_flux/numerical_flux_euler_T.c:60-327, 470-778
+//   inverse mass (GL/GL diag)   solvers/dg/solve_dg.c:265-282, volume_solver_T.c:115-148
+//   SSP-RK3                     solvers/solve_explicit.c:252-301
+//   positivity                  solvers/solve.c:187-233, 356-374; solution/solution.c:81-95
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dgx {
+
+constexpr double kGamma = 1.4;
+constexpr double kGm1 = 0.4;
+constexpr double kEpsPhys = 1.0e-13;
+
+// 1-D operators of the GL/GL collocated discretization for orders up to 7 (8 points).
+struct Ops1D {
+    double w[8];        // GL weights
+    double D[64];       // D[q*n + a] = l_a'(x_q)
+    double l[2][8];     // l_a(-1), l_a(+1)
+    double sb[64];      // nodal -> Bezier (ccSB0), sb[i*n + k]
+    double bs[64];      // Bezier -> nodal (ccBS0)
+};
+
+template <int D, int P>
+struct Shape {
+    static constexpr int N1 = P + 1;
+    static constexpr int NS = (D == 2) ? N1 * N1 : N1 * N1 * N1;
+    static constexpr int NF = (D == 2) ? N1 : N1 * N1;
+    static constexpr int NFACE = 2 * D;
+    static constexpr int NV = D + 2;
+    static constexpr int BLOCK = 256;
+    static constexpr int EPB = BLOCK / NS;             // elements per block
+    static constexpr int NSP = (NS <= 16) ? 16 : (NS <= 32) ? 32 : (NS <= 64) ? 64 : (NS <= 128) ? 128 : 256;
+    static_assert(EPB >= 1, "element too large for one block");
+};
+
+// Packed neighbour record: bits 0-27 neighbour element, 28-30 its local face, 31 = 1 when this
+// element is the face's right volume (neigh_info[1]).
+__host__ __device__ inline uint32_t pack_nbr(uint32_t e, uint32_t lf, uint32_t right)
+{
+    return (e & 0x0FFFFFFFu) | (lf << 28) | (right << 31);
+}
+
+template <int D, int P>
+__device__ inline int face_node(int dn, int a, int m)
+{
+    constexpr int N1 = P + 1;
+    if constexpr (D == 2) {
+        return dn == 0 ? a + N1 * m : m + N1 * a;
+    } else {
+        const int m0 = m % N1, m1 = m / N1;
+        if (dn == 0) return a + N1 * (m0 + N1 * m1);
+        if (dn == 1) return m0 + N1 * (a + N1 * m1);
+        return m0 + N1 * (m1 + N1 * a);
+    }
+}
+
+template <int D, int P>
+__device__ inline double face_weight(const Ops1D& op, int m)
+{
+    constexpr int N1 = P + 1;
+    if constexpr (D == 2)
+        return op.w[m];
+    else
+        return op.w[m % N1] * op.w[m / N1];
+}
+
+// Euler flux F_k(eq) contracted with a direction: out[eq] = sum_k n_k F_k(eq)  (flux_euler_T.c:259-300)
+template <int D>
+__device__ inline void euler_flux_n(const double* w, const double* n, double* out)
+{
+    const double rho = w[0], E = w[D + 1];
+    const double rho_inv = 1.0 / rho;
+    double uvw[3] = { 0, 0, 0 }, V2 = 0.0, Vn = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        uvw[k] = rho_inv * w[1 + k];
+        V2 += uvw[k] * uvw[k];
+        Vn += n[k] * uvw[k];
+    }
+    const double p = kGm1 * (E - 0.5 * rho * V2);
+    out[0] = rho * Vn;
+#pragma unroll
+    for (int m = 0; m < D; ++m)
+        out[1 + m] = w[1 + m] * Vn + n[m] * p;
+    out[D + 1] = (E + p) * Vn;
+}
+
+// Lax-Friedrichs: 0.5 [n.(F_l + F_r) + max(|V|+c) (w_l - w_r)]  (numerical_flux_euler_T.c:100-181)
+template <int D>
+__device__ inline void numflux_lf(const double* wl, const double* wr, const double* n, double* nf)
+{
+    double lam[2];
+    const double* ws[2] = { wl, wr };
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const double* w = ws[s];
+        const double rho = w[0], rho_inv = 1.0 / rho;
+        double V2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double u = w[1 + k] * rho_inv;
+            V2 += u * u;
+        }
+        const double p = kGm1 * (w[D + 1] - 0.5 * rho * V2);
+        lam[s] = sqrt(V2) + sqrt(kGamma * p / rho);
+    }
+    const double maxV = (lam[0] > lam[1]) ? lam[0] : lam[1];
+    double fl[D + 2], fr[D + 2];
+    euler_flux_n<D>(wl, n, fl);
+    euler_flux_n<D>(wr, n, fr);
+#pragma unroll
+    for (int eq = 0; eq < D + 2; ++eq)
+        nf[eq] = 0.5 * ((fl[eq] + fr[eq]) + maxV * (wl[eq] - wr[eq]));
+}
+
+__device__ inline double max_abs_real(double a, double b)
+{
+    double c = (fabs(a) > fabs(b)) ? a : b;
+    return (c < 0.0) ? -c : c;
+}
+
+__device__ inline double min_abs_real(double a, double b)
+{
+    double c = (fabs(a) < fabs(b)) ? a : b;
+    return (c < 0.0) ? -c : c;
+}
+
+// Roe-Pike with the entropy fix of Qu et al. 2015 eq. (35)  (numerical_flux_euler_T.c:470-603)
+template <int D>
+__device__ inline void numflux_roe(const double* wl, const double* wr, const double* n, double* nf)
+{
+    double ul[3] = { 0, 0, 0 }, ur[3] = { 0, 0, 0 };
+    const double rhoL = wl[0], rhoR = wr[0], EL = wl[D + 1], ER = wr[D + 1];
+    const double rhoL_inv = 1.0 / rhoL, rhoR_inv = 1.0 / rhoR;
+    double V2L = 0, V2R = 0, VnL = 0, VnR = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        ul[k] = wl[1 + k] * rhoL_inv;
+        ur[k] = wr[1 + k] * rhoR_inv;
+        V2L += ul[k] * ul[k];
+        V2R += ur[k] * ur[k];
+        VnL += n[k] * ul[k];
+        VnR += n[k] * ur[k];
+    }
+    const double pL = kGm1 * (EL - 0.5 * rhoL * V2L), pR = kGm1 * (ER - 0.5 * rhoR * V2R);
+    const double HL = (EL + pL) * rhoL_inv, HR = (ER + pR) * rhoR_inv;
+    const double r = sqrt(rhoR / rhoL), rP1_inv = 1.0 / (r + 1.0);
+    const double rho = r * rhoL;
+    double u[3] = { 0, 0, 0 }, Vn = 0, V2 = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        u[k] = (r * ur[k] + ul[k]) * rP1_inv;
+        Vn += n[k] * u[k];
+        V2 += u[k] * u[k];
+    }
+    const double H = (r * HR + HL) * rP1_inv;
+    const double c2 = kGm1 * (H - 0.5 * V2);
+    const double c = sqrt(c2), c_inv = 1.0 / c;
+    const double l1 = min_abs_real(VnL - c, Vn - c);
+    const double l5 = max_abs_real(VnR + c, Vn + c);
+    const double l234 = (Vn > 0.0) ? Vn : -Vn;
+    const double drho = rhoR - rhoL, dE = ER - EL, dp = pR - pL, dVn = VnR - VnL;
+    const double lc1 = 0.5 * (l5 + l1) - l234, lc2 = 0.5 * (l5 - l1);
+    const double disInter1 = lc1 * dp * (c_inv * c_inv) + lc2 * rho * dVn * c_inv;
+    const double disInter2 = lc1 * rho * dVn + lc2 * dp * c_inv;
+    const double rhoVnL = rhoL * VnL, rhoVnR = rhoR * VnR, pLR = pL + pR;
+    nf[0] = 0.5 * ((rhoVnL + rhoVnR) - (l234 * drho + disInter1));
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double nFk = rhoVnL * ul[k] + rhoVnR * ur[k] + n[k] * pLR;
+        const double drhou = rhoR * ur[k] - rhoL * ul[k];
+        nf[1 + k] = 0.5 * (nFk - (l234 * drhou + disInter1 * u[k] + disInter2 * n[k]));
+    }
+    const double nFE = VnL * (EL + pL) + VnR * (ER + pR);
+    nf[D + 1] = 0.5 * (nFE - (l234 * dE + disInter1 * H + disInter2 * Vn));
+}
+
+template <int D, int FLUX>
+__device__ inline void numflux(const double* wl, const double* wr, const double* n, double* nf)
+{
+    if constexpr (FLUX == 0)
+        numflux_lf<D>(wl, wr, n, nf);
+    else
+        numflux_roe<D>(wl, wr, n, nf);
+}
+
+struct StageArgs {
+    double* u;              // [NV][E][NS] state, updated in place
+    double* up;             // [NV][E][NS] RK register (sol_coef_p)
+    const double* met;      // [D*D][E][NS] metrics, component c = i + D*j
+    const double* imass;    // [E][NS] 1/(w J)
+    const double* tr_in;    // [E][NFACE][NV][NF] traces of the current state
+    double* tr_out;         // [E][NFACE][NV][NF] traces of the updated state
+    const uint32_t* nbr;    // [E][NFACE] packed neighbour records
+    double* rhs_out;        // [NV][E][NS] residual output (residual mode only)
+    double* blockmax;       // [grid] max |rhs| of each block (NaN propagates)
+    unsigned long long* bad_elem;  // first element that failed the positivity check
+    int64_t n_elem;
+    double dt;
+    int stage;              // 0,1,2: SSP-RK3 stage; -1: residual only
+    int apply_minv;         // residual mode: 1 = M^-1 applied
+    int positivity;
+};
+
+}  // namespace dgx

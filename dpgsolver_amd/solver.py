@@ -1,0 +1,149 @@
+"""Host-side mirror of the reference's explicit solver entry points, over the C-ABI of include/dgx.h.
+
+Reference entry points and their counterparts here:
+  compute_rhs(sim)                  src/simulation/solvers/solve.c:133-149          -> Solver.compute_rhs
+  time_step_ssp_rk_33 (one stage)   src/simulation/solvers/solve_explicit.c:252-301 -> Solver.rk_stage
+  solve_explicit(sim)               src/simulation/solvers/solve_explicit.c:78-118  -> Solver.solve_explicit
+  enforce_positivity_highorder      src/simulation/solvers/solve.c:187-233          (inside every stage)
+The reference aborts on errors (EXIT_ERROR); here every failure raises DgxError with the C status.
+All device work is done by libdgx.so; there is no other compute path.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _ffi
+
+NUM_FLUX = {"lax_friedrichs": _ffi.FLUX_LAX_FRIEDRICHS, "lf": _ffi.FLUX_LAX_FRIEDRICHS,
+            "roe_pike": _ffi.FLUX_ROE_PIKE, "roe": _ffi.FLUX_ROE_PIKE}
+PRECISION = {"fp64": _ffi.PREC_FP64, "mixed": _ffi.PREC_MIXED}
+
+
+class Solver:
+    """Explicit DG (Euler, QUAD/HEX, GL/GL collocated) on one GPU.
+
+    mesh, geo: dpgsolver_amd.problem.Mesh / Geometry; p: solution order; num_flux: "lf" | "roe";
+    positivity: apply enforce_positivity_highorder after every stage (the reference does for Euler,
+    test_case.c:63-79).
+    """
+
+    def __init__(self, mesh, geo, p, num_flux="roe", positivity=True, precision="fp64"):
+        self._lib = _ffi.load()
+        self.d, self.p = mesh.d, p
+        self.nv = mesh.d + 2
+        self.ns = (p + 1) ** mesh.d
+        self.n_elem = mesh.n_elem
+        cfg = _ffi.DgxConfig()
+        cfg.d, cfg.p, cfg.pde = mesh.d, p, 0
+        cfg.num_flux = NUM_FLUX[num_flux] if isinstance(num_flux, str) else int(num_flux)
+        cfg.positivity = 1 if positivity else 0
+        cfg.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
+        self._mesh_c = mesh.c_struct()
+        self._geo_c = geo.c_struct()
+        self._keep = (mesh, geo)
+        ctx = ctypes.c_void_p()
+        st = self._lib.dgx_create(ctypes.byref(cfg), ctypes.byref(self._mesh_c), ctypes.byref(self._geo_c),
+                                  ctypes.byref(ctx))
+        _ffi.check(st)
+        self._ctx = ctx
+
+    # -- lifetime ------------------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.dgx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        _ffi.check(st, self._ctx, self._lib)
+
+    @property
+    def shape(self):
+        return (self.n_elem, self.nv, self.ns)
+
+    @property
+    def stream(self):
+        return self._lib.dgx_get_stream(self._ctx)
+
+    # -- state ---------------------------------------------------------------------------------------
+    def set_state(self, u):
+        """u: numpy (host) or torch tensor (host or cuda) in the reference layout [E][nv][Ns]."""
+        ptr, kind, keep = _buffer(u, self.shape)
+        self._check(self._lib.dgx_set_state(self._ctx, _ffi.LAYOUT_REF, kind, ptr))
+        del keep
+
+    def get_state(self, out=None):
+        out = np.empty(self.shape) if out is None else out
+        ptr, kind, keep = _buffer(out, self.shape, writable=True)
+        self._check(self._lib.dgx_get_state(self._ctx, _ffi.LAYOUT_REF, kind, ptr))
+        return out
+
+    # -- residual and stepping -----------------------------------------------------------------------
+    def compute_rhs(self, apply_minv=True, precision="fp64", out=None):
+        """Residual of the current state; returns (rhs [E][nv][Ns], max_rhs)."""
+        out = np.empty(self.shape) if out is None else out
+        ptr, kind, keep = _buffer(out, self.shape, writable=True)
+        mx = ctypes.c_double(0.0)
+        self._check(self._lib.dgx_rhs(self._ctx, PRECISION[precision], int(bool(apply_minv)), _ffi.LAYOUT_REF,
+                                      kind, ptr, ctypes.byref(mx)))
+        return out, mx.value
+
+    def rk_stage(self, stage, dt):
+        mx = ctypes.c_double(0.0)
+        self._check(self._lib.dgx_rk_stage(self._ctx, int(stage), float(dt), ctypes.byref(mx)))
+        return mx.value
+
+    def step(self, dt, nsteps=1):
+        """nsteps SSP-RK3 steps as hipGraph launches; returns max_rhs of the last stage."""
+        mx = ctypes.c_double(0.0)
+        self._check(self._lib.dgx_step(self._ctx, float(dt), int(nsteps), ctypes.byref(mx)))
+        return mx.value
+
+    def solve_explicit(self, dt, time_final, exit_tol=None):
+        """Time loop of solve_explicit (solve_explicit.c:78-118): fixed dt, the last step clamped to
+        reach time_final; stops early when max_rhs < exit_tol."""
+        t = 0.0
+        mx = float("inf")
+        while t < time_final * (1 - 1e-14):
+            h = min(dt, time_final - t)
+            mx = self.step(h, 1)
+            t += h
+            if exit_tol is not None and mx < exit_tol:
+                break
+        return t, mx
+
+    def sync(self):
+        elem = ctypes.c_int64(-1)
+        st = self._lib.dgx_sync(self._ctx, ctypes.byref(elem))
+        self._check(st)
+
+    @property
+    def last_elapsed_ms(self):
+        return self._lib.dgx_last_elapsed_ms(self._ctx)
+
+
+def _buffer(a, shape, writable=False):
+    """(pointer, DGX_PTR_*, keep-alive) for a numpy array or torch tensor of `shape`, float64, contiguous."""
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        torch = None
+    if torch is not None and isinstance(a, torch.Tensor):
+        if a.dtype != torch.float64 or not a.is_contiguous() or tuple(a.shape) != tuple(shape):
+            raise ValueError(f"expected a contiguous float64 tensor of shape {shape}")
+        kind = _ffi.PTR_DEVICE if a.is_cuda else _ffi.PTR_HOST
+        return ctypes.c_void_p(a.data_ptr()), kind, a
+    a_np = a
+    if not isinstance(a_np, np.ndarray) or a_np.dtype != np.float64 or not a_np.flags["C_CONTIGUOUS"]:
+        if writable:
+            raise ValueError("output must be a contiguous float64 numpy array")
+        a_np = np.ascontiguousarray(a, dtype=np.float64)
+    if tuple(a_np.shape) != tuple(shape):
+        raise ValueError(f"expected shape {shape}, got {a_np.shape}")
+    return ctypes.c_void_p(a_np.ctypes.data), _ffi.PTR_HOST, a_np

@@ -1,0 +1,128 @@
+/*
+ * dgx.h - C-ABI of the MI355X explicit DG residual path (volume + face residual, inverse mass,
+ * SSP-RK3 stage with positivity enforcement) for the Euler equations on QUAD/HEX meshes.
+ *
+ * Each entry point replaces one reference entry point on the explicit path; the citation is given at
+ * each declaration.  Conventions that differ from the reference on purpose:
+ *   - every function returns a dgx_status and never aborts (the reference calls abort() via EXIT_*);
+ *   - there is no global state: all device memory and the HIP stream belong to a dgx_ctx
+ *     (one context per host thread at a time);
+ *   - host arrays are caller-owned and copied; nothing here keeps a pointer to caller memory.
+ *
+ * Host state layout DGX_LAYOUT_REF is the reference's per-volume sol_coef: for element e the
+ * (Ns, nv) column-major block, i.e. u[e][v][n] (src/simulation/solvers/volume_solver_T.h:31-104).
+ * Node n = i + (p+1)*(j + (p+1)*k) (r fastest), variables (rho, rho u, rho v, [rho w], E).
+ */
+#ifndef DGX_H
+#define DGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dgx_status {
+    DGX_OK = 0,
+    DGX_ERR_INVALID_ARG = 1,      /* bad pointer, size or unsupported option */
+    DGX_ERR_HIP = 2,              /* a HIP runtime call failed (message in dgx_last_error) */
+    DGX_ERR_POSITIVITY = 3,       /* element-average density or pressure < EPS_PHYS (solve.c:204-208) */
+    DGX_ERR_NONFINITE = 4,        /* NaN/Inf in the residual */
+    DGX_ERR_GEOMETRY = 5,         /* det J <= 0 (geometry_T.c:537-543) */
+    DGX_ERR_NO_DEVICE = 6         /* no GPU or the device code is missing */
+} dgx_status;
+
+enum { DGX_PDE_EULER = 0 };
+enum { DGX_FLUX_LAX_FRIEDRICHS = 0, DGX_FLUX_ROE_PIKE = 1 };   /* numerical_flux_euler_T.c:60-327, 470-778 */
+enum { DGX_PREC_FP64 = 0, DGX_PREC_MIXED = 1 };
+enum { DGX_LAYOUT_REF = 0, DGX_LAYOUT_SOA = 1 };               /* SOA = device layout u[v][e][n] */
+enum { DGX_PTR_HOST = 0, DGX_PTR_DEVICE = 1 };
+
+typedef struct dgx_config {
+    int32_t d;            /* 2 (QUAD) or 3 (HEX) */
+    int32_t p;            /* solution order; GL solution nodes = GL cubature, p+1 per direction */
+    int32_t pde;          /* DGX_PDE_EULER */
+    int32_t num_flux;     /* DGX_FLUX_* */
+    int32_t positivity;   /* 1: enforce_positivity_highorder after every RK stage (solve.c:187-233) */
+    int32_t precision;    /* DGX_PREC_* used by dgx_step / dgx_rk_stage */
+} dgx_config;
+
+typedef struct dgx_mesh {
+    int32_t d;
+    int32_t p_g;                 /* geometry order; GLL geometry nodes (p_g+1)^d per element */
+    int64_t n_elem;
+    int64_t n_face;
+    const double*  xyz_g;        /* [n_elem][d][(p_g+1)^d] (reference geom_coef, Lagrange at GLL) */
+    const int32_t* face_elem;    /* [n_face][2]  neigh_info[0/1].volume (face.h:47-95) */
+    const int8_t*  face_lf;      /* [n_face][2]  neigh_info[0/1].ind_lf */
+    const int8_t*  face_ord;     /* [n_face]     neigh_info[1].ind_ord; only 0 (aligned) is supported */
+} dgx_mesh;
+
+typedef struct dgx_geometry {
+    const double* metrics_vc;    /* [n_elem][d][d][Nc]  m(n,i,j) = J dxi_j/dx_i  (metrics_vc) */
+    const double* jdet_vc;       /* [n_elem][Nc]        (jacobian_det_vc) */
+    const double* normals_fc;    /* [n_face][Nf][d]     (normals_fc, from the left volume) */
+    const double* jdet_fc;       /* [n_face][Nf]        (jacobian_det_fc) */
+} dgx_geometry;
+
+typedef struct dgx_ctx dgx_ctx;
+
+/* Create a context on the current HIP device: validates the inputs, copies mesh connectivity and
+ * geometry to device memory in the SoA layouts the kernels read, builds the 1-D operators for p.
+ * Replaces the explicit-path part of constructor_derived_computational_elements(IL_SOLVER_DG)
+ * (solve_explicit.c:88-89 -> dg/volume_solver_dg_T.c:49-78: sol_coef_p, m_inv) and
+ * set_up_solver_geometry's consumers. */
+dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geometry* geom, dgx_ctx** out);
+dgx_status dgx_destroy(dgx_ctx* ctx);
+const char* dgx_last_error(const dgx_ctx* ctx);
+const char* dgx_status_string(dgx_status s);
+
+/* Returns the hipStream_t every call of this context is ordered on. */
+void* dgx_get_stream(dgx_ctx* ctx);
+
+/* Sizes: n_elem * nv * Ns doubles for a state. */
+int64_t dgx_state_size(const dgx_ctx* ctx);
+
+/* Upload / download the state (sol_coef of every volume).  ptr_kind: DGX_PTR_HOST or DGX_PTR_DEVICE
+ * (a device pointer must hold the same layout).  Uploading also refreshes the face traces. */
+dgx_status dgx_set_state(dgx_ctx* ctx, int layout, int ptr_kind, const double* u);
+dgx_status dgx_get_state(dgx_ctx* ctx, int layout, int ptr_kind, double* u);
+
+/* Residual of the current state.  apply_minv = 1: rhs = M^-1 (volume + face) as compute_rhs
+ * (solve.c:133-149 -> dg/solve_dg.c:130-136, 191-201); apply_minv = 0: the pre-M^-1 residual
+ * (volume + face), the operator test_support_solve_dg.c:164-184 linearizes.
+ * max_rhs (nullable) = max over volumes of ||rhs||_inf (compute_rlhs.c:59-71).  rhs (nullable) in
+ * `layout` on the host or device.  precision: DGX_PREC_*. */
+dgx_status dgx_rhs(dgx_ctx* ctx, int precision, int apply_minv, int layout, int ptr_kind, double* rhs,
+                   double* max_rhs);
+
+/* One SSP-RK3 stage (stage 0,1,2) of time_step_ssp_rk_33 (solve_explicit.c:252-301): residual,
+ * stage combination, then enforce_positivity_highorder (solve.c:187-233) when enabled.
+ * max_rhs (nullable) as for dgx_rhs, of this stage's residual. */
+dgx_status dgx_rk_stage(dgx_ctx* ctx, int stage, double dt, double* max_rhs);
+
+/* nsteps full SSP-RK3 steps with fixed dt, launched as a captured hipGraph.  max_rhs (nullable) of
+ * the last stage.  Replaces the stepping loop of solve_explicit (solve_explicit.c:78-118). */
+dgx_status dgx_step(dgx_ctx* ctx, double dt, int nsteps, double* max_rhs);
+
+/* Wait for all work on the context's stream and report a sticky device-side error
+ * (positivity failure with the first failing element in *elem, or a non-finite residual). */
+dgx_status dgx_sync(dgx_ctx* ctx, int64_t* elem);
+
+/* Residual linearization at the current state, on R = pre-M^-1 residual (apply_minv = 0) or M^-1 R
+ * (apply_minv = 1).  in/out in `layout`, host or device pointers.
+ *   dgx_vjp: out = J^T v   (hand-written adjoint kernels; cubature states are recomputed)
+ *   dgx_jvp: out = J w     (forward-mode kernels)
+ * Positivity enforcement is not part of R (test_support_solve_dg.c:164-184). */
+dgx_status dgx_vjp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* v, double* out);
+dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* w, double* out);
+
+/* Time of the last dgx_step / dgx_rk_stage kernel work in milliseconds, from HIP events recorded on
+ * the context stream around the launches (0 when unavailable). */
+double dgx_last_elapsed_ms(const dgx_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

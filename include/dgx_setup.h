@@ -1,0 +1,101 @@
+/*
+ * dgx_setup.h - host-side (C99) set-up producers for the explicit DG residual path.
+ *
+ * These functions produce the inputs that the solver entry points in dgx.h consume: reference-element
+ * nodes and 1-D operators, a structured periodic QUAD/HEX mesh with optional isoparametric curving,
+ * the geometry factors (cofactor metrics, det J, face normals and face Jacobians) and initial states.
+ * They play the role of the reference's set-up chain
+ *   constructor_Simulation -> constructor_derived_Solver_Element -> set_up_solver_geometry -> set_initial_solution
+ * (src/simulation/simulation.c:128-144, src/simulation/solvers/element_solver.c:179-300,
+ *  src/simulation/geometry/geometry_T.c:165-325,734-804, src/simulation/test_case/solution/solution_T.c:127-156).
+ *
+ * All arrays are caller-allocated; sizes follow from the integer parameters as documented. No function
+ * here allocates memory that outlives the call, touches a GPU, or aborts: errors come back as dgx_status.
+ */
+#ifndef DGX_SETUP_H
+#define DGX_SETUP_H
+
+#include <stdint.h>
+#include "dgx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Node families of the 1-D reference line (src/element/nodes.c:70-130: GL = "GLe", GLL = "GLoL"). */
+enum { DGX_NODES_GL = 0, DGX_NODES_GLL = 1 };
+
+/* 1-D nodes and weights on [-1,1], n = p+1 points (GLL requires p >= 1).  x[n], w[n]. */
+dgx_status dgx_nodes_1d(int family, int p, double* x, double* w);
+
+/* 1-D Lagrange basis on nodes xs[ns] evaluated (der = 0) or differentiated (der = 1) at points y[ny]:
+ * out[i*ns + j] = l_j^(der)(y_i)  (row-major ny x ns). */
+dgx_status dgx_lagrange_1d(int ns, const double* xs, int ny, const double* y, int der, double* out);
+
+/* The 1-D operator set the kernels use for a GL solution/cubature order p (collocated GL/GL):
+ *   x[n], w[n]                       GL nodes and weights, n = p+1
+ *   D[n*n]   D[i*n + a] = l_a'(x_i)  derivative of the solution basis at the cubature nodes
+ *   lm[n], lp[n]                     l_a(-1), l_a(+1)  (face trace rows; cv0_vs_fc for a LINE)
+ *   sb[n*n]  ccSB0 (nodal -> Bezier coefficients), row-major
+ *   bs[n*n]  ccBS0 (Bezier -> nodal), row-major
+ * Reference: the LINE sub-operators that element_operators_tp.c:90-241 Kronecker-multiplies
+ * (cv0_vs_vc, cv1_vt_vc, cv0_vs_fc, w_vc, ccSB0_vs_vs, ccBS0_vs_vs). */
+typedef struct dgx_ops1d {
+    int32_t p;
+    int32_t n;
+    double x[8], w[8];
+    double D[64];
+    double lm[8], lp[8];
+    double sb[64], bs[64];
+} dgx_ops1d;
+
+dgx_status dgx_ops1d_build(int p, dgx_ops1d* ops);
+
+/* Number of elements / faces / geometry nodes of a structured periodic mesh with n[0..d-1] cells. */
+int64_t dgx_mesh_structured_n_elem(int d, const int* n);
+int64_t dgx_mesh_structured_n_face(int d, const int* n);
+
+/* Structured periodic QUAD (d=2) / HEX (d=3) mesh on the box [lo,hi].
+ * Element e = ix + n0*(iy + n1*iz); faces f = dir*n_elem + e join e (left, local face 2*dir+1) to its
+ * periodic +dir neighbour (right, local face 2*dir).  Geometry nodes are GLL of order p_g per direction,
+ * ordered r fastest.  When amp != 0 every node is displaced by
+ *     x_k += amp * h_k * prod_{m != k} sin(x_m + phi[k][m]),   phi[k][m] = 2*pi*U(seed)
+ * evaluated at the undisplaced position (integer wavenumber, so the curving is periodic on [0,2pi]^d).
+ * Outputs: xyz_g[n_elem][d][(p_g+1)^d], face_elem[n_face][2], face_lf[n_face][2], face_ord[n_face]. */
+dgx_status dgx_mesh_structured(int d, const int* n, const double* lo, const double* hi, int p_g,
+                               double amp, uint64_t seed,
+                               double* xyz_g, int32_t* face_elem, int8_t* face_lf, int8_t* face_ord);
+
+/* Geometry factors for solution order p on GL cubature (collocated GL/GL), following
+ * geometry_T.c:734-804 (metric terms at the GLL metric nodes, interpolated to the cubature nodes),
+ * geometry_T.c:484-590 (det J; 2-D cofactors) and geometry_T.c:412-445,653-659 (normals, face det J);
+ * the 3-D cofactors use the conservative curl form of deprecated/src/setup_geom_factors.c:182-250.
+ * Outputs (reference layouts):
+ *   metrics_vc[n_elem][d][d][Nc]   per element (Nc,d,d) column-major: m(n,i,j) = J dxi_j/dx_i
+ *   jdet_vc[n_elem][Nc]
+ *   normals_fc[n_face][Nf][d]      unit normals, outward from the face's left element
+ *   jdet_fc[n_face][Nf]            |C n_ref|
+ * Returns DGX_ERR_GEOMETRY when det J <= 0 at any cubature node (geometry_T.c:537-543). */
+dgx_status dgx_geometry_build(const dgx_mesh* mesh, int p,
+                              double* metrics_vc, double* jdet_vc, double* normals_fc, double* jdet_fc);
+
+/* Physical coordinates of the solution nodes: xyz_s[n_elem][d][Ns] (interpolated from the geometry nodes). */
+dgx_status dgx_solution_nodes_xyz(const dgx_mesh* mesh, int p, double* xyz_s);
+
+/* Initial states in the reference per-element layout u[n_elem][nv][Ns] (conservative variables).
+ *  - isentropic vortex (Shu): free stream rho=u=v=p=1 (2-D), strength beta, centred at (xc,yc);
+ *  - Taylor-Green vortex: rho=1, u=sin x cos y cos z, v=-cos x sin y cos z, w=0,
+ *    p = 1/(gamma M^2) + (cos 2x + cos 2y)(cos 2z + 2)/16 (3-D). */
+dgx_status dgx_ic_isentropic_vortex(const dgx_mesh* mesh, int p, double beta, double xc, double yc, double* u);
+dgx_status dgx_ic_taylor_green(const dgx_mesh* mesh, int p, double mach, double* u);
+
+/* Fixed time step from the initial state: cfl * h_min / ((2p+1) * d * max(|V| + c)). */
+double dgx_dt_cfl(int d, int p, int64_t n_elem, const double* u, double h_min, double cfl);
+
+/* Counter-based normal deviates: out[i] = N(0,1) from splitmix64(seed, i) via Box-Muller. */
+void dgx_randn(uint64_t seed, int64_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,87 @@
+/*
+ * oracle.h - CPU restatement of DPGSolver's explicit DG residual path.  TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker the GPU path is compared against.  Only tests/, the smoke check and
+ * bench.py's cpu_baseline leg may load it; the product library never links or calls it.
+ *
+ * It restates the reference algorithm as the reference executes it, with dense per-element operators
+ * (src/simulation/solvers/solve.c:133-149 -> dg/solve_dg.c:130-136, 182-201, 265-282):
+ *   volume:   s_vc = cv0_vs_vc u; f = Euler flux; fr = metrics . f; rhs += sum_j tw1_vt_vc[j] fr_j
+ *             (compute_volume_rlhs_T.c:145-177, 277-285; compute_rlhs_T.c:71-110; flux_euler_T.c:71-143)
+ *   face:     s_l = cv0_vs_fc[lf_l] u_l, s_r = cv0_vs_fc[lf_r] u_r; nnf = LF | Roe-Pike; nnf *= |J_f|;
+ *             rhs_l -= tw0_vt_fc[lf_l] nnf; rhs_r -= tw0_vt_fc[lf_r] (-nnf)
+ *             (compute_face_rlhs_T.c:51-146, 297-359; boundary_T.c:42-92,134-145;
+ *              numerical_flux_euler_T.c:60-327, 470-778)
+ *   inverse mass: rhs = m_inv rhs, m = V^T diag(w J) V, m_inv by LU (volume_solver_T.c:115-148)
+ *   SSP-RK3 + positivity (solve_explicit.c:252-301; solve.c:187-233, 356-374)
+ * Operators are built like the reference's: orthonormal Legendre basis, a Vandermonde solve for the
+ * Lagrange/Bezier bases, Kronecker products for QUAD/HEX (element_operators_tp.c:133-241).
+ * Geometry is recomputed here from the geometry nodes the same way (geometry_T.c:208-325, 412-445,
+ * 484-590, 653-659, 734-804; 3-D curl form deprecated/src/setup_geom_factors.c:182-250).
+ *
+ * Every function of this file is pinned by a check in tests/ (see each test's docstring): golden
+ * vectors from the reference's unit-test data, closed forms, invariants and complex-step identities.
+ */
+#ifndef DGX_ORACLE_H
+#define DGX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+/* 1-D nodes by Golub-Welsch (eigenvalues of the Jacobi matrix): family 0 = GL, 1 = GLL. */
+int orc_nodes_1d(int family, int p, double* x, double* w);
+
+/* Orthonormal-basis route to the Lagrange operators: out = Phi^(der)(y) Phi(xs)^-1 (ny x ns, row-major). */
+int orc_lagrange_1d(int ns, const double* xs, int ny, const double* y, int der, double* out);
+
+/* Dense Kronecker product of d row-major factors: op[(i0 + m0 (i1 + m1 i2)), (j0 + n0 (j1 + n1 j2))]. */
+int orc_kron(int d, const int* m, const int* n, const double* const* A, double* op);
+
+/* General dense matrix product C = A B (row-major), the reference's mm_d (matrix_math_T.c:178-201). */
+void orc_mm(int m, int k, int n, const double* A, const double* B, double* C);
+
+/* Context: mesh as in dgx_mesh, solution order p, flux 0 = LF / 1 = Roe-Pike, positivity 0/1. */
+orc_ctx* orc_create(int d, int p, int p_g, int64_t n_elem, int64_t n_face, const double* xyz_g,
+                    const int32_t* face_elem, const int8_t* face_lf, int num_flux, int positivity);
+void orc_destroy(orc_ctx* c);
+
+/* Geometry factors computed by the oracle, in the dgx_geometry layouts. */
+void orc_geometry(const orc_ctx* c, double* metrics_vc, double* jdet_vc, double* normals_fc, double* jdet_fc);
+
+/* Replace the oracle's geometry by given factors (same layouts) and rebuild the mass matrices: the
+ * residual path is then compared on exactly the geometry the GPU path consumes. */
+void orc_set_geometry(orc_ctx* c, const double* metrics_vc, const double* jdet_vc, const double* normals_fc,
+                      const double* jdet_fc);
+
+/* Dense element operators (for fixture checks): name = "cv0_vs_vc", "tw1_vt_vc0".., "cv0_vs_fc0"..,
+ * "tw0_vt_fc0"..,"ccSB0", "ccBS0", "w_vc"; returns the number of values written (or -1). */
+int orc_operator(const orc_ctx* c, const char* name, double* out);
+
+/* rhs = [M^-1] (volume + face) of u (layout u[e][v][n]); returns max_e ||rhs_e||_inf. */
+double orc_residual(orc_ctx* c, const double* u, double* rhs, int apply_minv);
+
+/* Complex-step directional derivative of the residual: out = Im R(u + i h w) / h. */
+void orc_jvp_cs(orc_ctx* c, const double* u, const double* w, double h, double* out, int apply_minv);
+
+/* Apply enforce_positivity_highorder to every element of u in place; returns -1 on success or the
+ * first element whose average density/pressure is below EPS_PHYS. */
+int64_t orc_positivity(orc_ctx* c, double* u);
+
+/* One SSP-RK3 stage on (u, u_p) in place; returns max_rhs of the stage's residual.  *bad_elem is set
+ * to the first positivity failure (or -1). */
+double orc_rk_stage(orc_ctx* c, int stage, double dt, double* u, double* u_p, int64_t* bad_elem);
+
+/* Physical flux / numerical flux at one point (d = ctx d): used by the flux unit tests.
+ * f[dim + d*eq] = F_dim(eq); nf[eq]. */
+void orc_flux_point(int d, const double* u, double* f);
+void orc_numflux_point(int d, int num_flux, const double* ul, const double* ur, const double* n, double* nf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
