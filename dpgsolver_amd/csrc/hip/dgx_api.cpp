@@ -1,5 +1,9 @@
 // C-ABI of include/dgx.h: context, uploads, and the stage/residual entry points.
-// Host code only; every device computation is a kernel in dgx_stage.hip.
+// Host code only; every device computation is a kernel in dgx_stage.hip / dgx_hex3.inc.
+//
+// Device arrays are in "slot" order: elements are grouped (2x2x2 bricks when the mesh allows it, for
+// the HEX p=3 group kernel; the identity otherwise) and slot = group*8 + position.  The caller only
+// sees element order; uploads and downloads permute.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -15,17 +19,24 @@
 struct dgx_ctx {
     int d = 0, p = 0, nv = 0, ns = 0, nf = 0, nface = 0;
     int flux = 0, positivity = 1, precision = 0;
+    bool hex3 = false;           // HEX p=3 group kernel
     int64_t E = 0, F = 0;
-    int64_t grid = 0;
+    int64_t Ep = 0;              // number of slots (E plus padding)
+    int64_t nmax = 0;            // entries of blockmax
     hipStream_t stream = nullptr;
-    double* u = nullptr;         // [nv][E][ns]
-    double* up = nullptr;        // [nv][E][ns]
-    double* rhs = nullptr;       // [nv][E][ns] residual / staging buffer
-    double* met = nullptr;       // [d*d][E][ns]
-    double* imass = nullptr;     // [E][ns]
-    double* tr[2] = { nullptr, nullptr };   // [E][nface][nv][nf]
-    uint32_t* nbr = nullptr;     // [E][nface]
-    double* blockmax = nullptr;  // [grid]
+    double* u = nullptr;         // [nv][Ep][ns]
+    double* up = nullptr;        // [nv][Ep][ns]
+    double* rhs = nullptr;       // [nv][Ep][ns] residual / staging buffer
+    double* stage_buf = nullptr; // [E*nv*ns] caller-layout staging for host transfers
+    double* met = nullptr;       // [d*d][Ep][ns]
+    double* imass = nullptr;     // [Ep][ns]
+    double* tr[2] = { nullptr, nullptr };   // [Ep][nface][nv][nf]
+    uint32_t* nbr = nullptr;     // [Ep][nface]
+    int32_t* slot_elem = nullptr;   // [Ep]
+    int32_t* elem_slot = nullptr;   // [E]
+    uint8_t* face_list = nullptr;   // [G][48]
+    uint8_t* face_count = nullptr;  // [G]
+    double* blockmax = nullptr;  // [nmax]
     double* dmax = nullptr;      // [1]
     unsigned long long* bad_elem = nullptr;
     unsigned int* nonfinite = nullptr;
@@ -60,8 +71,9 @@ void free_all(dgx_ctx* c)
             (void)hipGraphExecDestroy(c->graph[i]);
         c->graph[i] = nullptr;
     }
-    void* ptrs[] = { c->u, c->up, c->rhs, c->met, c->imass, c->tr[0], c->tr[1], c->nbr, c->blockmax,
-                     c->dmax, c->bad_elem, c->nonfinite };
+    void* ptrs[] = { c->u, c->up, c->rhs, c->stage_buf, c->met, c->imass, c->tr[0], c->tr[1], c->nbr,
+                     c->slot_elem, c->elem_slot, c->face_list, c->face_count, c->blockmax, c->dmax,
+                     c->bad_elem, c->nonfinite };
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -97,69 +109,133 @@ dgx_status check_flags(dgx_ctx* c, int64_t* elem)
     return DGX_OK;
 }
 
-dgx::StageArgs base_args(dgx_ctx* c)
+// Launches one stage (stage 0..2) or the residual (stage -1) reading traces tr[tin].
+hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin)
 {
+    if (c->hex3) {
+        dgx::Hex3Args a;
+        std::memset(&a, 0, sizeof(a));
+        a.u = c->u;
+        a.up = c->up;
+        a.met = c->met;
+        a.imass = c->imass;
+        a.tr_in = c->tr[tin];
+        a.tr_out = c->tr[tin ^ 1];
+        a.nbr = c->nbr;
+        a.slot_elem = c->slot_elem;
+        a.face_list = c->face_list;
+        a.face_count = c->face_count;
+        a.rhs_out = c->rhs;
+        a.blockmax = c->blockmax;
+        a.bad_elem = c->bad_elem;
+        a.n_slots = c->Ep;
+        a.dt = dt;
+        a.stage = stage;
+        a.apply_minv = apply_minv;
+        a.positivity = c->positivity;
+        return dgx::launch_hex3_stage(c->flux, a, c->stream);
+    }
     dgx::StageArgs a;
     std::memset(&a, 0, sizeof(a));
     a.u = c->u;
     a.up = c->up;
     a.met = c->met;
     a.imass = c->imass;
-    a.tr_in = c->tr[c->cur];
-    a.tr_out = c->tr[c->cur ^ 1];
+    a.tr_in = c->tr[tin];
+    a.tr_out = c->tr[tin ^ 1];
     a.nbr = c->nbr;
     a.rhs_out = c->rhs;
     a.blockmax = c->blockmax;
     a.bad_elem = c->bad_elem;
     a.n_elem = c->E;
+    a.dt = dt;
+    a.stage = stage;
+    a.apply_minv = apply_minv;
     a.positivity = c->positivity;
-    return a;
+    return dgx::launch_stage(c->d, c->p, c->flux, a, c->stream);
 }
 
 dgx_status read_max(dgx_ctx* c, double* max_rhs)
 {
-    HIP_TRY(c, dgx::launch_reduce_max(c->blockmax, c->grid, c->dmax, c->nonfinite, c->stream));
+    HIP_TRY(c, dgx::launch_reduce_max(c->blockmax, c->nmax, c->dmax, c->nonfinite, c->stream));
     if (max_rhs)
         HIP_TRY(c, hipMemcpyAsync(max_rhs, c->dmax, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     return DGX_OK;
 }
 
-// Converts between a caller buffer and the device SoA state buffer dst/src.
-dgx_status upload_state(dgx_ctx* c, int layout, int ptr_kind, const double* src, double* dst_soa)
+// caller buffer (element order, `layout`) -> internal slot-order buffer dst
+dgx_status upload_state(dgx_ctx* c, int layout, int ptr_kind, const double* src, double* dst)
 {
     const size_t bytes = sizeof(double) * (size_t)c->E * c->nv * c->ns;
-    if (layout == DGX_LAYOUT_SOA) {
-        HIP_TRY(c, hipMemcpyAsync(dst_soa, src, bytes, ptr_kind == DGX_PTR_HOST ? hipMemcpyHostToDevice
-                                                                                 : hipMemcpyDeviceToDevice,
-                                  c->stream));
-        return DGX_OK;
-    }
-    const double* ref = src;
+    const double* s = src;
     if (ptr_kind == DGX_PTR_HOST) {
-        double* stage = (dst_soa == c->rhs) ? c->up : c->rhs;
-        HIP_TRY(c, hipMemcpyAsync(stage, src, bytes, hipMemcpyHostToDevice, c->stream));
-        ref = stage;
+        HIP_TRY(c, hipMemcpyAsync(c->stage_buf, src, bytes, hipMemcpyHostToDevice, c->stream));
+        s = c->stage_buf;
     }
-    HIP_TRY(c, dgx::launch_ref_to_soa(ref, dst_soa, c->E, c->nv, c->ns, c->stream));
+    HIP_TRY(c, dgx::launch_to_internal(s, layout, dst, c->slot_elem, c->E, c->Ep, c->nv, c->ns, c->stream));
     return DGX_OK;
 }
 
-dgx_status download_state(dgx_ctx* c, int layout, int ptr_kind, const double* src_soa, double* dst, double* scratch)
+// internal slot-order buffer src -> caller buffer (element order, `layout`)
+dgx_status download_state(dgx_ctx* c, int layout, int ptr_kind, const double* src, double* dst)
 {
     const size_t bytes = sizeof(double) * (size_t)c->E * c->nv * c->ns;
-    if (layout == DGX_LAYOUT_SOA) {
-        HIP_TRY(c, hipMemcpyAsync(dst, src_soa, bytes, ptr_kind == DGX_PTR_HOST ? hipMemcpyDeviceToHost
-                                                                                 : hipMemcpyDeviceToDevice,
-                                  c->stream));
-        return DGX_OK;
-    }
-    if (ptr_kind == DGX_PTR_DEVICE) {
-        HIP_TRY(c, dgx::launch_soa_to_ref(src_soa, dst, c->E, c->nv, c->ns, c->stream));
-        return DGX_OK;
-    }
-    HIP_TRY(c, dgx::launch_soa_to_ref(src_soa, scratch, c->E, c->nv, c->ns, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(dst, scratch, bytes, hipMemcpyDeviceToHost, c->stream));
+    double* d = (ptr_kind == DGX_PTR_HOST) ? c->stage_buf : dst;
+    HIP_TRY(c, dgx::launch_from_internal(src, d, layout, c->elem_slot, c->E, c->Ep, c->nv, c->ns, c->stream));
+    if (ptr_kind == DGX_PTR_HOST)
+        HIP_TRY(c, hipMemcpyAsync(dst, c->stage_buf, bytes, hipMemcpyDeviceToHost, c->stream));
     return DGX_OK;
+}
+
+// Groups of up to 8 elements: 2x2x2 bricks found through the +r/+s/+t neighbours, the remaining
+// elements packed in index order.  Returns slot_elem (size multiple of 8, -1 = padding).
+std::vector<int32_t> build_groups(int64_t E, const std::vector<int64_t>& nb /* [E][6] */)
+{
+    std::vector<char> used((size_t)E, 0);
+    std::vector<int32_t> slots;
+    slots.reserve((size_t)E + 8);
+    int64_t scan = 0;
+    for (int64_t e = 0; e < E; ++e) {
+        if (used[e])
+            continue;
+        int64_t b[8];
+        b[0] = e;
+        b[1] = nb[e * 6 + 1];
+        b[2] = nb[e * 6 + 3];
+        b[3] = nb[b[1] * 6 + 3];
+        bool ok = nb[b[2] * 6 + 1] == b[3];
+        for (int q = 0; q < 4 && ok; ++q)
+            b[4 + q] = nb[b[q] * 6 + 5];
+        ok = ok && nb[b[4] * 6 + 1] == b[5] && nb[b[4] * 6 + 3] == b[6] && nb[b[5] * 6 + 3] == b[7] &&
+             nb[b[6] * 6 + 1] == b[7];
+        for (int q = 0; q < 8 && ok; ++q) {
+            if (used[b[q]])
+                ok = false;
+            for (int r = 0; r < q && ok; ++r)
+                if (b[r] == b[q])
+                    ok = false;
+        }
+        if (ok) {
+            for (int q = 0; q < 8; ++q) {
+                used[b[q]] = 1;
+                slots.push_back((int32_t)b[q]);
+            }
+            continue;
+        }
+        // packed group of the next unused elements
+        int cnt = 0;
+        if (scan < e)
+            scan = e;
+        for (; scan < E && cnt < 8; ++scan)
+            if (!used[scan]) {
+                used[scan] = 1;
+                slots.push_back((int32_t)scan);
+                ++cnt;
+            }
+        while (cnt++ < 8)
+            slots.push_back(-1);
+    }
+    return slots;
 }
 
 }  // namespace
@@ -197,7 +273,7 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         (cfg->num_flux != DGX_FLUX_LAX_FRIEDRICHS && cfg->num_flux != DGX_FLUX_ROE_PIKE) ||
         (cfg->precision != DGX_PREC_FP64 && cfg->precision != DGX_PREC_MIXED))
         return DGX_ERR_INVALID_ARG;
-    if (mesh->n_elem < 1 || mesh->n_elem >= (1 << 28) || mesh->n_face < 1 || !mesh->face_elem || !mesh->face_lf ||
+    if (mesh->n_elem < 1 || mesh->n_elem >= (1 << 27) || mesh->n_face < 1 || !mesh->face_elem || !mesh->face_lf ||
         !mesh->face_ord || !geom->metrics_vc || !geom->jdet_vc)
         return DGX_ERR_INVALID_ARG;
 
@@ -205,49 +281,98 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return DGX_ERR_NO_DEVICE;
 
+    const int n1 = p + 1;
+    const int ns = (d == 2) ? n1 * n1 : n1 * n1 * n1;
+    const int nface = 2 * d;
+    const int64_t E = mesh->n_elem, F = mesh->n_face;
+
+    // neighbour table in element order: every (element, local face) slot must be filled exactly once
+    std::vector<int64_t> nbe((size_t)E * nface, -1);
+    std::vector<int8_t> nbl((size_t)E * nface, -1), nbr_right((size_t)E * nface, 0);
+    for (int64_t f = 0; f < F; ++f) {
+        const int64_t eL = mesh->face_elem[2 * f], eR = mesh->face_elem[2 * f + 1];
+        const int lL = mesh->face_lf[2 * f], lR = mesh->face_lf[2 * f + 1];
+        if (eL < 0 || eL >= E || eR < 0 || eR >= E || lL < 0 || lL >= nface || lR < 0 || lR >= nface)
+            return DGX_ERR_INVALID_ARG;
+        if (mesh->face_ord[f] != 0)
+            return DGX_ERR_INVALID_ARG;   // only aligned node correspondence is supported
+        const size_t sl = eL * nface + lL, sr = eR * nface + lR;
+        if (nbe[sl] >= 0 || nbe[sr] >= 0)
+            return DGX_ERR_INVALID_ARG;
+        nbe[sl] = eR;
+        nbl[sl] = (int8_t)lR;
+        nbr_right[sl] = 0;
+        nbe[sr] = eL;
+        nbl[sr] = (int8_t)lL;
+        nbr_right[sr] = 1;
+    }
+    for (int64_t v : nbe)
+        if (v < 0)
+            return DGX_ERR_INVALID_ARG;   // boundary faces are not part of the periodic explicit path
+
     dgx_ctx* c = new dgx_ctx();
     c->d = d;
     c->p = p;
     c->nv = d + 2;
-    const int n1 = p + 1;
-    c->ns = (d == 2) ? n1 * n1 : n1 * n1 * n1;
+    c->ns = ns;
     c->nf = (d == 2) ? n1 : n1 * n1;
-    c->nface = 2 * d;
+    c->nface = nface;
     c->flux = cfg->num_flux;
     c->positivity = cfg->positivity ? 1 : 0;
     c->precision = cfg->precision;
-    c->E = mesh->n_elem;
-    c->F = mesh->n_face;
-    c->grid = dgx::stage_grid(d, p, c->E);
-    const int64_t E = c->E, ns = c->ns;
+    c->E = E;
+    c->F = F;
+    c->hex3 = (d == 3 && p == 3);
 
-    // neighbour table: every (element, local face) slot must be filled exactly once
-    std::vector<uint32_t> nbr((size_t)E * c->nface, 0xFFFFFFFFu);
-    for (int64_t f = 0; f < c->F; ++f) {
-        const int64_t eL = mesh->face_elem[2 * f], eR = mesh->face_elem[2 * f + 1];
-        const int lL = mesh->face_lf[2 * f], lR = mesh->face_lf[2 * f + 1];
-        if (eL < 0 || eL >= E || eR < 0 || eR >= E || lL < 0 || lL >= c->nface || lR < 0 || lR >= c->nface) {
-            delete c;
-            return DGX_ERR_INVALID_ARG;
-        }
-        if (mesh->face_ord[f] != 0) {
-            delete c;   // only aligned node correspondence is supported
-            return DGX_ERR_INVALID_ARG;
-        }
-        uint32_t& sl = nbr[eL * c->nface + lL];
-        uint32_t& sr = nbr[eR * c->nface + lR];
-        if (sl != 0xFFFFFFFFu || sr != 0xFFFFFFFFu) {
-            delete c;
-            return DGX_ERR_INVALID_ARG;
-        }
-        sl = dgx::pack_nbr((uint32_t)eR, (uint32_t)lR, 0u);
-        sr = dgx::pack_nbr((uint32_t)eL, (uint32_t)lL, 1u);
+    std::vector<int32_t> slot_elem;
+    if (c->hex3) {
+        slot_elem = build_groups(E, nbe);
+    } else {
+        slot_elem.resize((size_t)E);
+        for (int64_t e = 0; e < E; ++e)
+            slot_elem[e] = (int32_t)e;
     }
-    for (uint32_t v : nbr)
-        if (v == 0xFFFFFFFFu) {
-            delete c;   // boundary faces are not part of the periodic explicit path
-            return DGX_ERR_INVALID_ARG;
+    const int64_t Ep = (int64_t)slot_elem.size();
+    c->Ep = Ep;
+    std::vector<int32_t> elem_slot((size_t)E, -1);
+    for (int64_t s = 0; s < Ep; ++s)
+        if (slot_elem[s] >= 0)
+            elem_slot[slot_elem[s]] = (int32_t)s;
+
+    // neighbour records in slot order; group face lists
+    std::vector<uint32_t> nbr((size_t)Ep * nface, 0u);
+    for (int64_t s = 0; s < Ep; ++s) {
+        const int64_t e = slot_elem[s];
+        if (e < 0)
+            continue;
+        for (int f = 0; f < nface; ++f) {
+            const size_t k = e * nface + f;
+            nbr[s * nface + f] = dgx::pack_nbr((uint32_t)elem_slot[nbe[k]], (uint32_t)nbl[k], (uint32_t)nbr_right[k]);
         }
+    }
+    std::vector<uint8_t> flist, fcount;
+    if (c->hex3) {
+        const int64_t G = Ep / 8;
+        flist.assign((size_t)G * 48, 0);
+        fcount.assign((size_t)G, 0);
+        for (int64_t g = 0; g < G; ++g) {
+            int cnt = 0;
+            for (int sl = 0; sl < 8; ++sl) {
+                const int64_t s = g * 8 + sl;
+                if (slot_elem[s] < 0)
+                    continue;
+                for (int f = 0; f < nface; ++f) {
+                    const uint32_t rec = nbr[s * nface + f];
+                    const int64_t nsl = rec & 0x0FFFFFFFu;
+                    const bool right = (rec >> 31) != 0;
+                    const bool local = (nsl / 8) == g;
+                    if (!local || !right)
+                        flist[g * 48 + cnt++] = (uint8_t)(sl * 6 + f);
+                }
+            }
+            fcount[g] = (uint8_t)cnt;
+        }
+    }
 
     dgx_ops1d o;
     if (dgx_ops1d_build(p, &o) != DGX_OK) {
@@ -267,11 +392,14 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         ops.bs[i] = o.bs[i];
     }
 
-    // metrics -> [d*d][E][ns]; inverse of the diagonal mass 1/(w J)
-    std::vector<double> met((size_t)d * d * E * ns), imass((size_t)E * ns);
-    for (int64_t e = 0; e < E; ++e) {
+    // metrics -> [d*d][Ep][ns]; inverse of the diagonal mass 1/(w J) -> [Ep][ns]
+    std::vector<double> met((size_t)d * d * Ep * ns, 0.0), imass((size_t)Ep * ns, 0.0);
+    for (int64_t s = 0; s < Ep; ++s) {
+        const int64_t e = slot_elem[s];
+        if (e < 0)
+            continue;
         for (int cc = 0; cc < d * d; ++cc)
-            std::memcpy(&met[((size_t)cc * E + e) * ns], geom->metrics_vc + ((size_t)e * d * d + cc) * ns,
+            std::memcpy(&met[((size_t)cc * Ep + s) * ns], geom->metrics_vc + ((size_t)e * d * d + cc) * ns,
                         sizeof(double) * ns);
         for (int n = 0; n < ns; ++n) {
             const int i0 = n % n1, i1 = (n / n1) % n1, i2 = n / (n1 * n1);
@@ -283,12 +411,13 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
                 delete c;
                 return DGX_ERR_GEOMETRY;
             }
-            imass[e * ns + n] = 1.0 / (W * J);
+            imass[s * ns + n] = 1.0 / (W * J);
         }
     }
 
-    const size_t sbytes = sizeof(double) * (size_t)E * c->nv * ns;
-    const size_t tbytes = sizeof(double) * (size_t)E * c->nface * c->nv * c->nf;
+    c->nmax = c->hex3 ? Ep : dgx::stage_grid(d, p, E);
+    const size_t sbytes = sizeof(double) * (size_t)Ep * c->nv * ns;
+    const size_t tbytes = sizeof(double) * (size_t)Ep * nface * c->nv * c->nf;
     dgx_status st = DGX_OK;
     auto step = [&](hipError_t e_, const char* what) {
         if (st == DGX_OK && e_ != hipSuccess)
@@ -300,12 +429,19 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     step(hipMalloc(&c->u, sbytes), "hipMalloc(u)");
     step(hipMalloc(&c->up, sbytes), "hipMalloc(up)");
     step(hipMalloc(&c->rhs, sbytes), "hipMalloc(rhs)");
+    step(hipMalloc(&c->stage_buf, sizeof(double) * (size_t)E * c->nv * ns), "hipMalloc(stage)");
     step(hipMalloc(&c->met, sizeof(double) * met.size()), "hipMalloc(met)");
     step(hipMalloc(&c->imass, sizeof(double) * imass.size()), "hipMalloc(imass)");
     step(hipMalloc(&c->tr[0], tbytes), "hipMalloc(tr0)");
     step(hipMalloc(&c->tr[1], tbytes), "hipMalloc(tr1)");
     step(hipMalloc(&c->nbr, sizeof(uint32_t) * nbr.size()), "hipMalloc(nbr)");
-    step(hipMalloc(&c->blockmax, sizeof(double) * c->grid), "hipMalloc(blockmax)");
+    step(hipMalloc(&c->slot_elem, sizeof(int32_t) * slot_elem.size()), "hipMalloc(slot_elem)");
+    step(hipMalloc(&c->elem_slot, sizeof(int32_t) * elem_slot.size()), "hipMalloc(elem_slot)");
+    if (c->hex3) {
+        step(hipMalloc(&c->face_list, flist.size()), "hipMalloc(face_list)");
+        step(hipMalloc(&c->face_count, fcount.size()), "hipMalloc(face_count)");
+    }
+    step(hipMalloc(&c->blockmax, sizeof(double) * c->nmax), "hipMalloc(blockmax)");
     step(hipMalloc(&c->dmax, sizeof(double)), "hipMalloc(dmax)");
     step(hipMalloc(&c->bad_elem, sizeof(unsigned long long)), "hipMalloc(bad)");
     step(hipMalloc(&c->nonfinite, sizeof(unsigned int)), "hipMalloc(nonfinite)");
@@ -313,18 +449,26 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         step(hipMemcpy(c->met, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice), "copy met");
         step(hipMemcpy(c->imass, imass.data(), sizeof(double) * imass.size(), hipMemcpyHostToDevice), "copy imass");
         step(hipMemcpy(c->nbr, nbr.data(), sizeof(uint32_t) * nbr.size(), hipMemcpyHostToDevice), "copy nbr");
+        step(hipMemcpy(c->slot_elem, slot_elem.data(), sizeof(int32_t) * slot_elem.size(), hipMemcpyHostToDevice),
+             "copy slot_elem");
+        step(hipMemcpy(c->elem_slot, elem_slot.data(), sizeof(int32_t) * elem_slot.size(), hipMemcpyHostToDevice),
+             "copy elem_slot");
+        if (c->hex3) {
+            step(hipMemcpy(c->face_list, flist.data(), flist.size(), hipMemcpyHostToDevice), "copy face_list");
+            step(hipMemcpy(c->face_count, fcount.data(), fcount.size(), hipMemcpyHostToDevice), "copy face_count");
+        }
         step(hipMemset(c->u, 0, sbytes), "memset u");
         step(hipMemset(c->up, 0, sbytes), "memset up");
+        step(hipMemset(c->rhs, 0, sbytes), "memset rhs");
         step(hipMemset(c->tr[0], 0, tbytes), "memset tr0");
         step(hipMemset(c->tr[1], 0, tbytes), "memset tr1");
-        step(hipMemset(c->blockmax, 0, sizeof(double) * c->grid), "memset blockmax");
+        step(hipMemset(c->blockmax, 0, sizeof(double) * c->nmax), "memset blockmax");
         step(dgx::upload_ops(p, ops), "upload ops");
     }
     if (st == DGX_OK)
         st = reset_flags(c);
-    if (st == DGX_OK) {
+    if (st == DGX_OK)
         step(hipStreamSynchronize(c->stream), "sync");
-    }
     if (st != DGX_OK) {
         fprintf(stderr, "dgx_create: %s\n", c->err.c_str());
         free_all(c);
@@ -358,14 +502,19 @@ int64_t dgx_state_size(const dgx_ctx* ctx)
 
 dgx_status dgx_set_state(dgx_ctx* ctx, int layout, int ptr_kind, const double* u)
 {
-    if (!ctx || !u || (layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    if (!u || (layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
         (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_set_state: bad arguments");
     dgx_status st = upload_state(ctx, layout, ptr_kind, u, ctx->u);
     if (st != DGX_OK)
         return st;
     ctx->cur = 0;
-    HIP_TRY(ctx, dgx::launch_trace(ctx->d, ctx->p, ctx->u, ctx->tr[0], ctx->E, ctx->stream));
+    if (ctx->hex3)
+        HIP_TRY(ctx, dgx::launch_hex3_trace(ctx->u, ctx->tr[0], ctx->slot_elem, ctx->Ep, ctx->stream));
+    else
+        HIP_TRY(ctx, dgx::launch_trace(ctx->d, ctx->p, ctx->u, ctx->tr[0], ctx->E, ctx->stream));
     st = reset_flags(ctx);
     if (st != DGX_OK)
         return st;
@@ -375,10 +524,12 @@ dgx_status dgx_set_state(dgx_ctx* ctx, int layout, int ptr_kind, const double* u
 
 dgx_status dgx_get_state(dgx_ctx* ctx, int layout, int ptr_kind, double* u)
 {
-    if (!ctx || !u || (layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    if (!u || (layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
         (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_get_state: bad arguments");
-    dgx_status st = download_state(ctx, layout, ptr_kind, ctx->u, u, ctx->rhs);
+    dgx_status st = download_state(ctx, layout, ptr_kind, ctx->u, u);
     if (st != DGX_OK)
         return st;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -388,22 +539,21 @@ dgx_status dgx_get_state(dgx_ctx* ctx, int layout, int ptr_kind, double* u)
 dgx_status dgx_rhs(dgx_ctx* ctx, int precision, int apply_minv, int layout, int ptr_kind, double* rhs,
                    double* max_rhs)
 {
-    if (!ctx || (precision != DGX_PREC_FP64 && precision != DGX_PREC_MIXED))
-        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: bad arguments");
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    if (precision != DGX_PREC_FP64 && precision != DGX_PREC_MIXED)
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: bad precision");
     if (rhs && ((layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
                 (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE)))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: bad layout");
-    dgx::StageArgs a = base_args(ctx);
-    a.stage = -1;
-    a.apply_minv = apply_minv ? 1 : 0;
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(ctx, dgx::launch_stage(ctx->d, ctx->p, ctx->flux, a, ctx->stream));
+    HIP_TRY(ctx, launch_any(ctx, -1, 0.0, apply_minv ? 1 : 0, ctx->cur));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     dgx_status st = read_max(ctx, max_rhs);
     if (st != DGX_OK)
         return st;
     if (rhs) {
-        st = download_state(ctx, layout, ptr_kind, ctx->rhs, rhs, ctx->up);
+        st = download_state(ctx, layout, ptr_kind, ctx->rhs, rhs);
         if (st != DGX_OK)
             return st;
     }
@@ -416,13 +566,12 @@ dgx_status dgx_rhs(dgx_ctx* ctx, int precision, int apply_minv, int layout, int 
 
 dgx_status dgx_rk_stage(dgx_ctx* ctx, int stage, double dt, double* max_rhs)
 {
-    if (!ctx || stage < 0 || stage > 2 || !std::isfinite(dt))
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    if (stage < 0 || stage > 2 || !std::isfinite(dt))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rk_stage: bad arguments");
-    dgx::StageArgs a = base_args(ctx);
-    a.stage = stage;
-    a.dt = dt;
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(ctx, dgx::launch_stage(ctx->d, ctx->p, ctx->flux, a, ctx->stream));
+    HIP_TRY(ctx, launch_any(ctx, stage, dt, 1, ctx->cur));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     ctx->cur ^= 1;
     dgx_status st = read_max(ctx, max_rhs);
@@ -446,12 +595,7 @@ static dgx_status build_graph(dgx_ctx* c, int parity, double dt)
     int cur = parity;
     hipError_t le = hipSuccess;
     for (int s = 0; s < 3 && le == hipSuccess; ++s) {
-        dgx::StageArgs a = base_args(c);
-        a.tr_in = c->tr[cur];
-        a.tr_out = c->tr[cur ^ 1];
-        a.stage = s;
-        a.dt = dt;
-        le = dgx::launch_stage(c->d, c->p, c->flux, a, c->stream);
+        le = launch_any(c, s, dt, 1, cur);
         cur ^= 1;
     }
     hipError_t ee = hipStreamEndCapture(c->stream, &g);
@@ -468,7 +612,9 @@ static dgx_status build_graph(dgx_ctx* c, int parity, double dt)
 
 dgx_status dgx_step(dgx_ctx* ctx, double dt, int nsteps, double* max_rhs)
 {
-    if (!ctx || nsteps < 0 || !std::isfinite(dt))
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    if (nsteps < 0 || !std::isfinite(dt))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_step: bad arguments");
     if (nsteps == 0)
         return DGX_OK;

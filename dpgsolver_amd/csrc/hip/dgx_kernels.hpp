@@ -219,4 +219,26 @@ struct StageArgs {
     int positivity;
 };
 
+// Arguments of the HEX p=3 group kernel; arrays are in slot order (slot = group*8 + wave).
+struct Hex3Args {
+    double* u;                  // [NV][Ep][64]
+    double* up;                 // [NV][Ep][64]
+    const double* met;          // [9][Ep][64]
+    const double* imass;        // [Ep][64]
+    const double* tr_in;        // [Ep][6][NV][16]
+    double* tr_out;             // [Ep][6][NV][16] (faces read by another group only)
+    const uint32_t* nbr;        // [Ep][6] packed neighbour records (slot of the neighbour)
+    const int32_t* slot_elem;   // [Ep] element of the slot, -1 for padding
+    const uint8_t* face_list;   // [G][48] faces (slot*6 + f) whose flux the group evaluates
+    const uint8_t* face_count;  // [G]
+    double* rhs_out;            // [NV][Ep][64] residual mode output
+    double* blockmax;           // [Ep] max |rhs| per element
+    unsigned long long* bad_elem;
+    int64_t n_slots;
+    double dt;
+    int stage;
+    int apply_minv;
+    int positivity;
+};
+
 }  // namespace dgx

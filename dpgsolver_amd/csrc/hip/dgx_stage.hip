@@ -500,28 +500,48 @@ __global__ __launch_bounds__(256) void trace_kernel(const double* u, double* tr,
     write_traces<D, P>(su, tr, e, n, valid);
 }
 
-// ref [E][NV][NS] <-> soa [NV][E][NS]
-__global__ void ref_to_soa_kernel(const double* ref, double* soa, int64_t E, int NV, int NS)
+// Caller layouts (element order): ref [E][NV][NS] (layout 0) or soa [NV][E][NS] (layout 1).
+// Internal state: [NV][Ep][NS] in slot order; padding slots hold zeros.
+__device__ inline int64_t caller_index(int layout, int64_t e, int64_t v, int64_t n, int64_t E, int NV, int NS)
 {
-    const int64_t total = E * NV * NS;
+    return layout == 0 ? (e * NV + v) * NS + n : (v * E + e) * NS + n;
+}
+
+__global__ void to_internal_kernel(const double* src, int layout, double* dst, const int32_t* slot_elem, int64_t E,
+                                   int64_t Ep, int NV, int NS)
+{
+    const int64_t total = Ep * NV * NS;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t n = i % NS, t = i / NS, e = t % E, v = t / E;   // i indexes soa
-        soa[i] = ref[(e * NV + v) * NS + n];
+        const int64_t n = i % NS, t = i / NS, s = t % Ep, v = t / Ep;
+        const int64_t e = slot_elem[s];
+        dst[i] = (e >= 0) ? src[caller_index(layout, e, v, n, E, NV, NS)] : 0.0;
     }
 }
 
-__global__ void soa_to_ref_kernel(const double* soa, double* ref, int64_t E, int NV, int NS)
+__global__ void from_internal_kernel(const double* src, double* dst, int layout, const int32_t* elem_slot, int64_t E,
+                                     int64_t Ep, int NV, int NS)
 {
     const int64_t total = E * NV * NS;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t n = i % NS, t = i / NS, v = t % NV, e = t / NV;   // i indexes ref
-        ref[i] = soa[(v * E + e) * NS + n];
+        int64_t e, v, n;
+        if (layout == 0) {
+            n = i % NS;
+            const int64_t t = i / NS;
+            v = t % NV;
+            e = t / NV;
+        } else {
+            n = i % NS;
+            const int64_t t = i / NS;
+            e = t % E;
+            v = t / E;
+        }
+        dst[i] = src[(v * Ep + elem_slot[e]) * NS + n];
     }
 }
 
 __global__ void reduce_max_kernel(const double* bm, int64_t nb, double* out, unsigned int* nonfinite)
 {
-    __shared__ double s[4];
+    __shared__ double s[16];
     double m = 0.0;
     for (int64_t i = threadIdx.x; i < nb; i += blockDim.x)
         m = nan_max(m, bm[i]);
@@ -635,21 +655,23 @@ hipError_t launch_trace(int d, int p, const double* u, double* tr, int64_t E, hi
 #undef DGX_CALL
 }
 
-hipError_t launch_ref_to_soa(const double* ref, double* soa, int64_t E, int nv, int ns, hipStream_t st)
+hipError_t launch_to_internal(const double* src, int layout, double* dst, const int32_t* slot_elem, int64_t E,
+                              int64_t Ep, int nv, int ns, hipStream_t st)
 {
-    ref_to_soa_kernel<<<dim3(2048), dim3(256), 0, st>>>(ref, soa, E, nv, ns);
+    to_internal_kernel<<<dim3(2048), dim3(256), 0, st>>>(src, layout, dst, slot_elem, E, Ep, nv, ns);
     return hipGetLastError();
 }
 
-hipError_t launch_soa_to_ref(const double* soa, double* ref, int64_t E, int nv, int ns, hipStream_t st)
+hipError_t launch_from_internal(const double* src, double* dst, int layout, const int32_t* elem_slot, int64_t E,
+                                int64_t Ep, int nv, int ns, hipStream_t st)
 {
-    soa_to_ref_kernel<<<dim3(2048), dim3(256), 0, st>>>(soa, ref, E, nv, ns);
+    from_internal_kernel<<<dim3(2048), dim3(256), 0, st>>>(src, dst, layout, elem_slot, E, Ep, nv, ns);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_max(const double* bm, int64_t nb, double* out, unsigned int* nonfinite, hipStream_t st)
 {
-    reduce_max_kernel<<<dim3(1), dim3(256), 0, st>>>(bm, nb, out, nonfinite);
+    reduce_max_kernel<<<dim3(1), dim3(1024), 0, st>>>(bm, nb, out, nonfinite);
     return hipGetLastError();
 }
 
@@ -660,3 +682,5 @@ hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, h
 }
 
 }  // namespace dgx
+
+#include "dgx_hex3.inc"
