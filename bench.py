@@ -162,7 +162,8 @@ def main():
                    "stages_per_step": 3, "precision": args.precision, "dt": dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"dgx::stage_kernel<{d},{p},{num_flux}>",
+                     "kernel": (f"dgx::hex3_stage_kernel<{num_flux}>" if (d, p) == (3, 3)
+                                else f"dgx::stage_kernel<{d},{p},{num_flux}>"),
                      "algorithmic_bytes_per_launch": bytes_alg, "bytes_per_element": b_elem,
                      "stage_us": t_stage_s * 1e6},
     }

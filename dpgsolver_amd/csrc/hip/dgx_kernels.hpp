@@ -81,12 +81,24 @@ __device__ inline double face_weight(const Ops1D& op, int m)
         return op.w[m % N1] * op.w[m / N1];
 }
 
+// 1/x from the hardware reciprocal estimate refined by two Newton steps (within ~1 ulp of the
+// correctly rounded quotient; the operands here are positive densities, sound speeds and face
+// Jacobians, so the scaling and special-value handling of a full IEEE division is not needed).
+__device__ inline double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Euler flux F_k(eq) contracted with a direction: out[eq] = sum_k n_k F_k(eq)  (flux_euler_T.c:259-300)
 template <int D>
 __device__ inline void euler_flux_n(const double* w, const double* n, double* out)
 {
     const double rho = w[0], E = w[D + 1];
-    const double rho_inv = 1.0 / rho;
+    const double rho_inv = rcp_nr(rho);
     double uvw[3] = { 0, 0, 0 }, V2 = 0.0, Vn = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -111,7 +123,7 @@ __device__ inline void numflux_lf(const double* wl, const double* wr, const doub
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         const double* w = ws[s];
-        const double rho = w[0], rho_inv = 1.0 / rho;
+        const double rho = w[0], rho_inv = rcp_nr(rho);
         double V2 = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
@@ -119,7 +131,7 @@ __device__ inline void numflux_lf(const double* wl, const double* wr, const doub
             V2 += u * u;
         }
         const double p = kGm1 * (w[D + 1] - 0.5 * rho * V2);
-        lam[s] = sqrt(V2) + sqrt(kGamma * p / rho);
+        lam[s] = sqrt(V2) + sqrt(kGamma * p * rho_inv);
     }
     const double maxV = (lam[0] > lam[1]) ? lam[0] : lam[1];
     double fl[D + 2], fr[D + 2];
@@ -148,7 +160,7 @@ __device__ inline void numflux_roe(const double* wl, const double* wr, const dou
 {
     double ul[3] = { 0, 0, 0 }, ur[3] = { 0, 0, 0 };
     const double rhoL = wl[0], rhoR = wr[0], EL = wl[D + 1], ER = wr[D + 1];
-    const double rhoL_inv = 1.0 / rhoL, rhoR_inv = 1.0 / rhoR;
+    const double rhoL_inv = rcp_nr(rhoL), rhoR_inv = rcp_nr(rhoR);
     double V2L = 0, V2R = 0, VnL = 0, VnR = 0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -161,7 +173,7 @@ __device__ inline void numflux_roe(const double* wl, const double* wr, const dou
     }
     const double pL = kGm1 * (EL - 0.5 * rhoL * V2L), pR = kGm1 * (ER - 0.5 * rhoR * V2R);
     const double HL = (EL + pL) * rhoL_inv, HR = (ER + pR) * rhoR_inv;
-    const double r = sqrt(rhoR / rhoL), rP1_inv = 1.0 / (r + 1.0);
+    const double r = sqrt(rhoR * rhoL_inv), rP1_inv = rcp_nr(r + 1.0);
     const double rho = r * rhoL;
     double u[3] = { 0, 0, 0 }, Vn = 0, V2 = 0;
 #pragma unroll
@@ -172,7 +184,7 @@ __device__ inline void numflux_roe(const double* wl, const double* wr, const dou
     }
     const double H = (r * HR + HL) * rP1_inv;
     const double c2 = kGm1 * (H - 0.5 * V2);
-    const double c = sqrt(c2), c_inv = 1.0 / c;
+    const double c = sqrt(c2), c_inv = rcp_nr(c);
     const double l1 = min_abs_real(VnL - c, Vn - c);
     const double l5 = max_abs_real(VnR + c, Vn + c);
     const double l234 = (Vn > 0.0) ? Vn : -Vn;
