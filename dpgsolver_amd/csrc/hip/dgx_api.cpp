@@ -36,6 +36,8 @@ struct dgx_ctx {
     int32_t* elem_slot = nullptr;   // [E]
     uint8_t* face_list = nullptr;   // [G][48]
     uint8_t* face_count = nullptr;  // [G]
+    double* lin_x = nullptr;     // [nv][Ep][ns] tangent / cotangent of dgx_jvp / dgx_vjp (allocated on first use)
+    double* lin_tr = nullptr;    // [Ep][nface][nv][nf] its traces
     double* blockmax = nullptr;  // [nmax]
     double* dmax = nullptr;      // [1]
     unsigned long long* bad_elem = nullptr;
@@ -73,7 +75,7 @@ void free_all(dgx_ctx* c)
     }
     void* ptrs[] = { c->u, c->up, c->rhs, c->stage_buf, c->met, c->imass, c->tr[0], c->tr[1], c->nbr,
                      c->slot_elem, c->elem_slot, c->face_list, c->face_count, c->blockmax, c->dmax,
-                     c->bad_elem, c->nonfinite };
+                     c->bad_elem, c->nonfinite, c->lin_x, c->lin_tr };
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -649,16 +651,63 @@ dgx_status dgx_sync(dgx_ctx* ctx, int64_t* elem)
     return check_flags(ctx, elem);
 }
 
+// J^T v or J w of the residual at the current state (pre-M^-1 residual, or M^-1 R with apply_minv).
+static dgx_status lin_apply(dgx_ctx* c, bool adjoint, int apply_minv, int layout, int ptr_kind, const double* in,
+                            double* out, const char* who)
+{
+    if (!in || !out || (layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
+        (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
+        return fail(c, DGX_ERR_INVALID_ARG, std::string(who) + ": bad arguments");
+    const size_t sbytes = sizeof(double) * (size_t)c->Ep * c->nv * c->ns;
+    const size_t tbytes = sizeof(double) * (size_t)c->Ep * c->nface * c->nv * c->nf;
+    if (!c->lin_x) {
+        HIP_TRY(c, hipMalloc(&c->lin_x, sbytes));
+        HIP_TRY(c, hipMalloc(&c->lin_tr, tbytes));
+    }
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    dgx_status st = upload_state(c, layout, ptr_kind, in, c->lin_x);
+    if (st != DGX_OK)
+        return st;
+    if (adjoint && apply_minv)
+        HIP_TRY(c, dgx::launch_scale_by_imass(c->lin_x, c->imass, c->lin_x, c->Ep, c->nv, c->ns, c->stream));
+    HIP_TRY(c, dgx::launch_trace(c->d, c->p, c->lin_x, c->lin_tr, c->Ep, c->stream));
+    dgx::LinArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.u = c->u;
+    a.x = c->lin_x;
+    a.met = c->met;
+    a.imass = c->imass;
+    a.tr_u = c->tr[c->cur];
+    a.tr_x = c->lin_tr;
+    a.nbr = c->nbr;
+    a.slot_elem = c->hex3 ? c->slot_elem : nullptr;
+    a.out = c->rhs;
+    a.n_elem = c->Ep;
+    a.apply_minv = (!adjoint && apply_minv) ? 1 : 0;
+    HIP_TRY(c, dgx::launch_lin(c->d, c->p, c->flux, adjoint, a, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    st = download_state(c, layout, ptr_kind, c->rhs, out);
+    if (st != DGX_OK)
+        return st;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess)
+        c->last_ms = ms;
+    return DGX_OK;
+}
+
 dgx_status dgx_vjp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* v, double* out)
 {
-    (void)apply_minv; (void)layout; (void)ptr_kind; (void)v; (void)out;
-    return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_vjp: not available yet");
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    return lin_apply(ctx, true, apply_minv, layout, ptr_kind, v, out, "dgx_vjp");
 }
 
 dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* w, double* out)
 {
-    (void)apply_minv; (void)layout; (void)ptr_kind; (void)w; (void)out;
-    return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_jvp: not available yet");
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    return lin_apply(ctx, false, apply_minv, layout, ptr_kind, w, out, "dgx_jvp");
 }
 
 double dgx_last_elapsed_ms(const dgx_ctx* ctx)

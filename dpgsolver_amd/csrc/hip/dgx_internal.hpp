@@ -20,6 +20,9 @@ hipError_t launch_from_internal(const double* src, double* dst, int layout, cons
 hipError_t launch_hex3_stage(int flux, const Hex3Args& args, hipStream_t st);
 hipError_t launch_hex3_trace(const double* u, double* tr, const int32_t* slot_elem, int64_t n_slots, hipStream_t st);
 hipError_t launch_reduce_max(const double* bm, int64_t nb, double* out, unsigned int* nonfinite, hipStream_t st);
+hipError_t launch_lin(int d, int p, int flux, bool adjoint, const LinArgs& args, hipStream_t st);
+hipError_t launch_scale_by_imass(const double* x, const double* imass, double* y, int64_t E, int nv, int ns,
+                                 hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
 
 }  // namespace dgx

@@ -81,6 +81,27 @@ __device__ inline double face_weight(const Ops1D& op, int m)
         return op.w[m % N1] * op.w[m / N1];
 }
 
+// Scalar types of the pointwise physics: double, and a forward-mode dual number (value + one tangent)
+// used by the linearized residual (JVP/VJP).  Comparisons and branch decisions use the value part,
+// as the reference's complex instantiation compares real parts (math_functions_T.c:222-236).
+struct Dual {
+    double v, d;
+};
+__device__ inline Dual operator+(Dual a, Dual b) { return { a.v + b.v, a.d + b.d }; }
+__device__ inline Dual operator-(Dual a, Dual b) { return { a.v - b.v, a.d - b.d }; }
+__device__ inline Dual operator-(Dual a) { return { -a.v, -a.d }; }
+__device__ inline Dual operator*(Dual a, Dual b) { return { a.v * b.v, fma(a.d, b.v, a.v * b.d) }; }
+__device__ inline Dual operator*(double a, Dual b) { return { a * b.v, a * b.d }; }
+__device__ inline Dual operator*(Dual a, double b) { return { a.v * b, a.d * b }; }
+__device__ inline Dual operator+(Dual a, double b) { return { a.v + b, a.d }; }
+__device__ inline Dual operator-(Dual a, double b) { return { a.v - b, a.d }; }
+__device__ inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
+
+__device__ inline double val(double x) { return x; }
+__device__ inline double val(Dual x) { return x.v; }
+__device__ inline double zero_like(double) { return 0.0; }
+__device__ inline Dual zero_like(Dual) { return { 0.0, 0.0 }; }
+
 // 1/x from the hardware reciprocal estimate refined by two Newton steps (within ~1 ulp of the
 // correctly rounded quotient; the operands here are positive densities, sound speeds and face
 // Jacobians, so the scaling and special-value handling of a full IEEE division is not needed).
@@ -92,21 +113,34 @@ __device__ inline double rcp_nr(double x)
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
 }
+__device__ inline Dual rcp_nr(Dual x)
+{
+    const double r = rcp_nr(x.v);
+    return { r, -x.d * r * r };
+}
+__device__ inline double sqrt_t(double x) { return sqrt(x); }
+__device__ inline Dual sqrt_t(Dual x)
+{
+    const double s = sqrt(x.v);
+    return { s, 0.5 * x.d * rcp_nr(s) };
+}
+__device__ inline double neg_if_negative(double x) { return x < 0.0 ? -x : x; }
+__device__ inline Dual neg_if_negative(Dual x) { return x.v < 0.0 ? -x : x; }
 
 // Euler flux F_k(eq) contracted with a direction: out[eq] = sum_k n_k F_k(eq)  (flux_euler_T.c:259-300)
-template <int D>
-__device__ inline void euler_flux_n(const double* w, const double* n, double* out)
+template <int D, class T, class N>
+__device__ inline void euler_flux_n(const T* w, const N* n, T* out)
 {
-    const double rho = w[0], E = w[D + 1];
-    const double rho_inv = rcp_nr(rho);
-    double uvw[3] = { 0, 0, 0 }, V2 = 0.0, Vn = 0.0;
+    const T rho = w[0], E = w[D + 1];
+    const T rho_inv = rcp_nr(rho);
+    T uvw[3], V2 = zero_like(rho), Vn = zero_like(rho);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         uvw[k] = rho_inv * w[1 + k];
-        V2 += uvw[k] * uvw[k];
-        Vn += n[k] * uvw[k];
+        V2 = V2 + uvw[k] * uvw[k];
+        Vn = Vn + n[k] * uvw[k];
     }
-    const double p = kGm1 * (E - 0.5 * rho * V2);
+    const T p = kGm1 * (E - 0.5 * rho * V2);
     out[0] = rho * Vn;
 #pragma unroll
     for (int m = 0; m < D; ++m)
@@ -115,26 +149,26 @@ __device__ inline void euler_flux_n(const double* w, const double* n, double* ou
 }
 
 // Lax-Friedrichs: 0.5 [n.(F_l + F_r) + max(|V|+c) (w_l - w_r)]  (numerical_flux_euler_T.c:100-181)
-template <int D>
-__device__ inline void numflux_lf(const double* wl, const double* wr, const double* n, double* nf)
+template <int D, class T>
+__device__ inline void numflux_lf(const T* wl, const T* wr, const double* n, T* nf)
 {
-    double lam[2];
-    const double* ws[2] = { wl, wr };
+    T lam[2];
+    const T* ws[2] = { wl, wr };
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const double* w = ws[s];
-        const double rho = w[0], rho_inv = rcp_nr(rho);
-        double V2 = 0.0;
+        const T* w = ws[s];
+        const T rho = w[0], rho_inv = rcp_nr(rho);
+        T V2 = zero_like(rho);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const double u = w[1 + k] * rho_inv;
-            V2 += u * u;
+            const T u = w[1 + k] * rho_inv;
+            V2 = V2 + u * u;
         }
-        const double p = kGm1 * (w[D + 1] - 0.5 * rho * V2);
-        lam[s] = sqrt(V2) + sqrt(kGamma * p * rho_inv);
+        const T p = kGm1 * (w[D + 1] - 0.5 * rho * V2);
+        lam[s] = sqrt_t(V2) + sqrt_t(kGamma * p * rho_inv);
     }
-    const double maxV = (lam[0] > lam[1]) ? lam[0] : lam[1];
-    double fl[D + 2], fr[D + 2];
+    const T maxV = (val(lam[0]) > val(lam[1])) ? lam[0] : lam[1];
+    T fl[D + 2], fr[D + 2];
     euler_flux_n<D>(wl, n, fl);
     euler_flux_n<D>(wr, n, fr);
 #pragma unroll
@@ -142,70 +176,72 @@ __device__ inline void numflux_lf(const double* wl, const double* wr, const doub
         nf[eq] = 0.5 * ((fl[eq] + fr[eq]) + maxV * (wl[eq] - wr[eq]));
 }
 
-__device__ inline double max_abs_real(double a, double b)
+// max_abs_real_T / min_abs_real_T (math_functions_T.c:222-236)
+template <class T>
+__device__ inline T max_abs_real(T a, T b)
 {
-    double c = (fabs(a) > fabs(b)) ? a : b;
-    return (c < 0.0) ? -c : c;
+    return neg_if_negative((fabs(val(a)) > fabs(val(b))) ? a : b);
 }
 
-__device__ inline double min_abs_real(double a, double b)
+template <class T>
+__device__ inline T min_abs_real(T a, T b)
 {
-    double c = (fabs(a) < fabs(b)) ? a : b;
-    return (c < 0.0) ? -c : c;
+    return neg_if_negative((fabs(val(a)) < fabs(val(b))) ? a : b);
 }
 
 // Roe-Pike with the entropy fix of Qu et al. 2015 eq. (35)  (numerical_flux_euler_T.c:470-603)
-template <int D>
-__device__ inline void numflux_roe(const double* wl, const double* wr, const double* n, double* nf)
+template <int D, class T>
+__device__ inline void numflux_roe(const T* wl, const T* wr, const double* n, T* nf)
 {
-    double ul[3] = { 0, 0, 0 }, ur[3] = { 0, 0, 0 };
-    const double rhoL = wl[0], rhoR = wr[0], EL = wl[D + 1], ER = wr[D + 1];
-    const double rhoL_inv = rcp_nr(rhoL), rhoR_inv = rcp_nr(rhoR);
-    double V2L = 0, V2R = 0, VnL = 0, VnR = 0;
+    const T zero = zero_like(wl[0]);
+    T ul[3], ur[3];
+    const T rhoL = wl[0], rhoR = wr[0], EL = wl[D + 1], ER = wr[D + 1];
+    const T rhoL_inv = rcp_nr(rhoL), rhoR_inv = rcp_nr(rhoR);
+    T V2L = zero, V2R = zero, VnL = zero, VnR = zero;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         ul[k] = wl[1 + k] * rhoL_inv;
         ur[k] = wr[1 + k] * rhoR_inv;
-        V2L += ul[k] * ul[k];
-        V2R += ur[k] * ur[k];
-        VnL += n[k] * ul[k];
-        VnR += n[k] * ur[k];
+        V2L = V2L + ul[k] * ul[k];
+        V2R = V2R + ur[k] * ur[k];
+        VnL = VnL + n[k] * ul[k];
+        VnR = VnR + n[k] * ur[k];
     }
-    const double pL = kGm1 * (EL - 0.5 * rhoL * V2L), pR = kGm1 * (ER - 0.5 * rhoR * V2R);
-    const double HL = (EL + pL) * rhoL_inv, HR = (ER + pR) * rhoR_inv;
-    const double r = sqrt(rhoR * rhoL_inv), rP1_inv = rcp_nr(r + 1.0);
-    const double rho = r * rhoL;
-    double u[3] = { 0, 0, 0 }, Vn = 0, V2 = 0;
+    const T pL = kGm1 * (EL - 0.5 * rhoL * V2L), pR = kGm1 * (ER - 0.5 * rhoR * V2R);
+    const T HL = (EL + pL) * rhoL_inv, HR = (ER + pR) * rhoR_inv;
+    const T r = sqrt_t(rhoR * rhoL_inv), rP1_inv = rcp_nr(r + 1.0);
+    const T rho = r * rhoL;
+    T u[3], Vn = zero, V2 = zero;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         u[k] = (r * ur[k] + ul[k]) * rP1_inv;
-        Vn += n[k] * u[k];
-        V2 += u[k] * u[k];
+        Vn = Vn + n[k] * u[k];
+        V2 = V2 + u[k] * u[k];
     }
-    const double H = (r * HR + HL) * rP1_inv;
-    const double c2 = kGm1 * (H - 0.5 * V2);
-    const double c = sqrt(c2), c_inv = rcp_nr(c);
-    const double l1 = min_abs_real(VnL - c, Vn - c);
-    const double l5 = max_abs_real(VnR + c, Vn + c);
-    const double l234 = (Vn > 0.0) ? Vn : -Vn;
-    const double drho = rhoR - rhoL, dE = ER - EL, dp = pR - pL, dVn = VnR - VnL;
-    const double lc1 = 0.5 * (l5 + l1) - l234, lc2 = 0.5 * (l5 - l1);
-    const double disInter1 = lc1 * dp * (c_inv * c_inv) + lc2 * rho * dVn * c_inv;
-    const double disInter2 = lc1 * rho * dVn + lc2 * dp * c_inv;
-    const double rhoVnL = rhoL * VnL, rhoVnR = rhoR * VnR, pLR = pL + pR;
+    const T H = (r * HR + HL) * rP1_inv;
+    const T c2 = kGm1 * (H - 0.5 * V2);
+    const T c = sqrt_t(c2), c_inv = rcp_nr(c);
+    const T l1 = min_abs_real(VnL - c, Vn - c);
+    const T l5 = max_abs_real(VnR + c, Vn + c);
+    const T l234 = (val(Vn) > 0.0) ? Vn : -Vn;
+    const T drho = rhoR - rhoL, dE = ER - EL, dp = pR - pL, dVn = VnR - VnL;
+    const T lc1 = 0.5 * (l5 + l1) - l234, lc2 = 0.5 * (l5 - l1);
+    const T disInter1 = lc1 * dp * (c_inv * c_inv) + lc2 * rho * dVn * c_inv;
+    const T disInter2 = lc1 * rho * dVn + lc2 * dp * c_inv;
+    const T rhoVnL = rhoL * VnL, rhoVnR = rhoR * VnR, pLR = pL + pR;
     nf[0] = 0.5 * ((rhoVnL + rhoVnR) - (l234 * drho + disInter1));
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        const double nFk = rhoVnL * ul[k] + rhoVnR * ur[k] + n[k] * pLR;
-        const double drhou = rhoR * ur[k] - rhoL * ul[k];
+        const T nFk = rhoVnL * ul[k] + rhoVnR * ur[k] + n[k] * pLR;
+        const T drhou = rhoR * ur[k] - rhoL * ul[k];
         nf[1 + k] = 0.5 * (nFk - (l234 * drhou + disInter1 * u[k] + disInter2 * n[k]));
     }
-    const double nFE = VnL * (EL + pL) + VnR * (ER + pR);
+    const T nFE = VnL * (EL + pL) + VnR * (ER + pR);
     nf[D + 1] = 0.5 * (nFE - (l234 * dE + disInter1 * H + disInter2 * Vn));
 }
 
-template <int D, int FLUX>
-__device__ inline void numflux(const double* wl, const double* wr, const double* n, double* nf)
+template <int D, int FLUX, class T>
+__device__ inline void numflux(const T* wl, const T* wr, const double* n, T* nf)
 {
     if constexpr (FLUX == 0)
         numflux_lf<D>(wl, wr, n, nf);
@@ -251,6 +287,21 @@ struct Hex3Args {
     int stage;
     int apply_minv;
     int positivity;
+};
+
+// Linearized residual (J w or J^T v) at the state u; arrays in the internal [v][slot][node] order.
+struct LinArgs {
+    const double* u;        // [NV][E][NS] linearization point
+    const double* x;        // [NV][E][NS] tangent w (JVP) or cotangent v (VJP; already scaled by 1/(wJ) when needed)
+    const double* met;      // [D*D][E][NS]
+    const double* imass;    // [E][NS]
+    const double* tr_u;     // [E][NFACE][NV][NF] traces of u
+    const double* tr_x;     // [E][NFACE][NV][NF] traces of x
+    const uint32_t* nbr;    // [E][NFACE]
+    const int32_t* slot_elem;   // [E] element of each slot (-1 = padding) or null when every slot is used
+    double* out;            // [NV][E][NS]
+    int64_t n_elem;
+    int apply_minv;         // JVP: scale the result by 1/(wJ)
 };
 
 }  // namespace dgx

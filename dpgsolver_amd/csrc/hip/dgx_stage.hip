@@ -684,3 +684,4 @@ hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, h
 }  // namespace dgx
 
 #include "dgx_hex3.inc"
+#include "dgx_lin.inc"

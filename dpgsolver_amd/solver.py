@@ -118,6 +118,26 @@ class Solver:
                 break
         return t, mx
 
+    # -- linearization -----------------------------------------------------------------------------
+    def jvp(self, w, apply_minv=False, out=None):
+        """J w, J = dR/du at the current state (R before M^-1, or M^-1 R with apply_minv); [E][nv][Ns]."""
+        return self._lin(self._lib.dgx_jvp, w, apply_minv, out)
+
+    def vjp(self, v, apply_minv=False, out=None):
+        """J^T v at the current state (same J as jvp); [E][nv][Ns]."""
+        return self._lin(self._lib.dgx_vjp, v, apply_minv, out)
+
+    def _lin(self, fn, x, apply_minv, out):
+        if out is None:
+            out = _like(x, self.shape)
+        pin, kin, keep_in = _buffer(x, self.shape)
+        pout, kout, keep_out = _buffer(out, self.shape, writable=True)
+        if kin != kout:
+            raise ValueError("input and output must both be host or both be device buffers")
+        self._check(fn(self._ctx, int(bool(apply_minv)), _ffi.LAYOUT_REF, kin, pin, pout))
+        del keep_in, keep_out
+        return out
+
     def sync(self):
         elem = ctypes.c_int64(-1)
         st = self._lib.dgx_sync(self._ctx, ctypes.byref(elem))
@@ -126,6 +146,50 @@ class Solver:
     @property
     def last_elapsed_ms(self):
         return self._lib.dgx_last_elapsed_ms(self._ctx)
+
+
+def _like(x, shape):
+    try:
+        import torch
+        if isinstance(x, torch.Tensor):
+            return torch.empty(shape, dtype=torch.float64, device=x.device)
+    except ImportError:  # pragma: no cover
+        pass
+    return np.empty(shape)
+
+
+def residual_function():
+    """torch.autograd.Function R(u) of a Solver: forward = dgx_rhs at u, backward = dgx_vjp.
+
+    Usage: R = residual_function(); r = R.apply(u, solver, apply_minv) with u a float64 cuda tensor
+    [E][nv][Ns] requiring grad; r.backward(g) gives u.grad = J^T g (J at u).  The solver's state is set
+    to u by the forward call.
+    """
+    import torch
+
+    class _Residual(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, u, solver, apply_minv=False):
+            u = u.detach().contiguous()
+            solver.set_state(u)
+            r = torch.empty_like(u)
+            solver.compute_rhs(apply_minv=apply_minv, out=r)
+            ctx.solver, ctx.apply_minv = solver, apply_minv
+            ctx.save_for_backward(u)
+            return r
+
+        @staticmethod
+        def backward(ctx, g):
+            (u,) = ctx.saved_tensors
+            s = ctx.solver
+            s.set_state(u)
+            return s.vjp(g.contiguous(), apply_minv=ctx.apply_minv), None, None
+
+        @staticmethod
+        def jvp(ctx, w, *_):
+            return ctx.solver.jvp(w.contiguous(), apply_minv=ctx.apply_minv)
+
+    return _Residual
 
 
 def _buffer(a, shape, writable=False):

@@ -34,7 +34,8 @@ struct orc_ctx {
     double* jdet_vc;            /* [E][Nc] */
     double* normals_fc;         /* [F][Nf][d] */
     double* jdet_fc;            /* [F][Nf] */
-    double* m_inv;              /* [E][Ns*Ns] */
+    double* m_inv;              /* [E][Ns*Ns], built on first use after the geometry is set */
+    int minv_ready;
 };
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -295,6 +296,8 @@ static int ipow(int b, int e)
 /* ------------------------------------------------------------------------------------------------ */
 /* templated residual (double and double complex)                                                    */
 /* ------------------------------------------------------------------------------------------------ */
+static void build_minv(orc_ctx* c);
+
 
 #define T double
 #define SFX(name) name##_d
@@ -333,7 +336,6 @@ void orc_numflux_point(int d, int num_flux, const double* ul, const double* ur, 
         numflux_roe_d(d, ul, ur, n, nf);
 }
 
-static void build_minv(orc_ctx* c);
 
 /* ------------------------------------------------------------------------------------------------ */
 /* context: operators, geometry, mass matrices                                                      */
@@ -507,8 +509,7 @@ static void build_geometry(orc_ctx* c, const double* xyz_g)
 orc_ctx* orc_create(int d, int p, int p_g, int64_t n_elem, int64_t n_face, const double* xyz_g,
                     const int32_t* face_elem, const int8_t* face_lf, int num_flux, int positivity)
 {
-    if ((d != 2 && d != 3) || p < 1 || p > 7 || p_g < 1 || p_g > 7 || n_elem < 1 || !xyz_g || !face_elem ||
-        !face_lf)
+    if ((d != 2 && d != 3) || p < 1 || p > 7 || p_g < 1 || p_g > 7 || n_elem < 1 || !face_elem || !face_lf)
         return NULL;
     orc_ctx* c = (orc_ctx*)calloc(1, sizeof(orc_ctx));
     c->d = d; c->p = p; c->p_g = p_g; c->num_flux = num_flux; c->positivity = positivity;
@@ -599,15 +600,14 @@ orc_ctx* orc_create(int d, int p, int p_g, int64_t n_elem, int64_t n_face, const
         c->w_vc[q] = wq;
     }
 
-    c->metrics_vc = (double*)malloc(sizeof(double) * (size_t)n_elem * d * d * Nc);
-    c->jdet_vc = (double*)malloc(sizeof(double) * (size_t)n_elem * Nc);
-    c->normals_fc = (double*)malloc(sizeof(double) * (size_t)n_face * Nf * d);
-    c->jdet_fc = (double*)malloc(sizeof(double) * (size_t)n_face * Nf);
-    build_geometry(c, xyz_g);
-
-    /* m = V^T diag(w J) V ; m_inv = m^-1 (volume_solver_T.c:115-148) */
-    c->m_inv = (double*)malloc(sizeof(double) * (size_t)n_elem * Ns * Ns);
-    build_minv(c);
+    c->metrics_vc = (double*)calloc((size_t)n_elem * d * d * Nc, sizeof(double));
+    c->jdet_vc = (double*)calloc((size_t)n_elem * Nc, sizeof(double));
+    c->normals_fc = (double*)calloc((size_t)n_face * Nf * d, sizeof(double));
+    c->jdet_fc = (double*)calloc((size_t)n_face * Nf, sizeof(double));
+    c->m_inv = (double*)calloc((size_t)n_elem * Ns * Ns, sizeof(double));
+    if (xyz_g)
+        build_geometry(c, xyz_g);
+    c->minv_ready = 0;   /* m = V^T diag(w J) V ; m_inv = m^-1 (volume_solver_T.c:115-148), built lazily */
     return c;
 }
 
@@ -656,6 +656,7 @@ static void build_minv(orc_ctx* c)
         invert(Ns, m, c->m_inv + (size_t)e * Ns * Ns);
     }
     free(m);
+    c->minv_ready = 1;
 }
 
 void orc_set_geometry(orc_ctx* c, const double* metrics_vc, const double* jdet_vc, const double* normals_fc,
@@ -666,7 +667,7 @@ void orc_set_geometry(orc_ctx* c, const double* metrics_vc, const double* jdet_v
     memcpy(c->jdet_vc, jdet_vc, sizeof(double) * (size_t)c->n_elem * c->Nc);
     memcpy(c->normals_fc, normals_fc, sizeof(double) * (size_t)c->n_face * c->Nf * d);
     memcpy(c->jdet_fc, jdet_fc, sizeof(double) * (size_t)c->n_face * c->Nf);
-    build_minv(c);
+    c->minv_ready = 0;
 }
 
 int orc_operator(const orc_ctx* c, const char* name, double* out)

@@ -45,7 +45,8 @@ int orc_kron(int d, const int* m, const int* n, const double* const* A, double* 
 /* General dense matrix product C = A B (row-major), the reference's mm_d (matrix_math_T.c:178-201). */
 void orc_mm(int m, int k, int n, const double* A, const double* B, double* C);
 
-/* Context: mesh as in dgx_mesh, solution order p, flux 0 = LF / 1 = Roe-Pike, positivity 0/1. */
+/* Context: mesh as in dgx_mesh, solution order p, flux 0 = LF / 1 = Roe-Pike, positivity 0/1.
+ * xyz_g may be NULL: the geometry is then left unset (zero) until orc_set_geometry provides it. */
 orc_ctx* orc_create(int d, int p, int p_g, int64_t n_elem, int64_t n_face, const double* xyz_g,
                     const int32_t* face_elem, const int8_t* face_lf, int num_flux, int positivity);
 void orc_destroy(orc_ctx* c);

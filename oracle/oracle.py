@@ -126,7 +126,9 @@ class Oracle:
         self.Ns = (p + 1) ** mesh.d
         self.Nf = (p + 1) ** (mesh.d - 1)
         self.nv = mesh.d + 2
-        self._ptr = lib.orc_create(mesh.d, p, mesh.p_g, mesh.n_elem, mesh.n_face, _d(mesh.xyz_g),
+        # with injected geometry the oracle's own (dense, slow) geometry construction is skipped
+        xyz = None if geometry is not None else _d(mesh.xyz_g)
+        self._ptr = lib.orc_create(mesh.d, p, mesh.p_g, mesh.n_elem, mesh.n_face, xyz,
                                    mesh.face_elem.ctypes.data_as(_i32p), mesh.face_lf.ctypes.data_as(_i8p),
                                    int(num_flux), int(bool(positivity)))
         if not self._ptr:
