@@ -539,11 +539,14 @@ __global__ void from_internal_kernel(const double* src, double* dst, int layout,
     }
 }
 
+// max over nb non-negative (or NaN) entries into *out, which must hold +0.0 before the launch: one
+// partial max per block, combined with a 64-bit integer atomic max (non-negative doubles order like
+// their bit patterns, and every NaN pattern compares above +inf).
 __global__ void reduce_max_kernel(const double* bm, int64_t nb, double* out, unsigned int* nonfinite)
 {
     __shared__ double s[16];
     double m = 0.0;
-    for (int64_t i = threadIdx.x; i < nb; i += blockDim.x)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (int64_t)gridDim.x * blockDim.x)
         m = nan_max(m, bm[i]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
@@ -555,9 +558,9 @@ __global__ void reduce_max_kernel(const double* bm, int64_t nb, double* out, uns
         double b = s[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
             b = nan_max(b, s[w]);
-        *out = b;
+        atomicMax(reinterpret_cast<unsigned long long*>(out), (unsigned long long)__double_as_longlong(b));
         if (!(b <= 1.0e308))
-            *nonfinite = 1u;
+            atomicOr(nonfinite, 1u);
     }
 }
 
@@ -671,7 +674,9 @@ hipError_t launch_from_internal(const double* src, double* dst, int layout, cons
 
 hipError_t launch_reduce_max(const double* bm, int64_t nb, double* out, unsigned int* nonfinite, hipStream_t st)
 {
-    reduce_max_kernel<<<dim3(1), dim3(1024), 0, st>>>(bm, nb, out, nonfinite);
+    fill_u64_kernel<<<dim3(1), dim3(64), 0, st>>>(reinterpret_cast<unsigned long long*>(out), 1, 0ull);
+    const int64_t blocks = (nb + 1023) / 1024;
+    reduce_max_kernel<<<dim3((unsigned)(blocks < 256 ? blocks : 256)), dim3(1024), 0, st>>>(bm, nb, out, nonfinite);
     return hipGetLastError();
 }
 
