@@ -27,6 +27,7 @@ STATUS_NAMES = {
     6: "DGX_ERR_NO_DEVICE",
 }
 FLUX_LAX_FRIEDRICHS, FLUX_ROE_PIKE = 0, 1
+PDE_EULER, PDE_NAVIER_STOKES = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 LAYOUT_REF, LAYOUT_SOA = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
@@ -35,7 +36,8 @@ NODES_GL, NODES_GLL = 0, 1
 
 class DgxConfig(ctypes.Structure):
     _fields_ = [("d", ctypes.c_int32), ("p", ctypes.c_int32), ("pde", ctypes.c_int32),
-                ("num_flux", ctypes.c_int32), ("positivity", ctypes.c_int32), ("precision", ctypes.c_int32)]
+                ("num_flux", ctypes.c_int32), ("positivity", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("mu", ctypes.c_double), ("Pr", ctypes.c_double)]
 
 
 class DgxMesh(ctypes.Structure):

@@ -19,6 +19,10 @@
 struct dgx_ctx {
     int d = 0, p = 0, nv = 0, ns = 0, nf = 0, nface = 0;
     int flux = 0, positivity = 1, precision = 0;
+    bool visc = false;           // Navier-Stokes (BR2) instead of Euler
+    double mu = 0.0, Pr = 1.0, eta = 0.0;
+    double* gtr = nullptr;       // NS: [Ep][nface][d*nv][nf] face traces of the volume gradient
+    double* lam = nullptr;       // NS: [Ep][nface][nf] lift trace factors
     bool hex3 = false;           // HEX p=3 group kernel
     int64_t E = 0, F = 0;
     int64_t Ep = 0;              // number of slots (E plus padding)
@@ -75,7 +79,7 @@ void free_all(dgx_ctx* c)
     }
     void* ptrs[] = { c->u, c->up, c->rhs, c->stage_buf, c->met, c->imass, c->tr[0], c->tr[1], c->nbr,
                      c->slot_elem, c->elem_slot, c->face_list, c->face_count, c->blockmax, c->dmax,
-                     c->bad_elem, c->nonfinite, c->lin_x, c->lin_tr };
+                     c->bad_elem, c->nonfinite, c->lin_x, c->lin_tr, c->gtr, c->lam };
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -154,6 +158,14 @@ hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin)
     a.stage = stage;
     a.apply_minv = apply_minv;
     a.positivity = c->positivity;
+    if (c->visc) {
+        a.gtr = c->gtr;
+        a.lam = c->lam;
+        a.mu = c->mu;
+        a.Pr = c->Pr;
+        a.eta = c->eta;
+        return dgx::launch_ns_stage(c->d, c->p, c->flux, a, c->gtr, c->stream);
+    }
     return dgx::launch_stage(c->d, c->p, c->flux, a, c->stream);
 }
 
@@ -271,7 +283,10 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     if (!cfg || !mesh || !geom)
         return DGX_ERR_INVALID_ARG;
     const int d = cfg->d, p = cfg->p;
-    if (!dgx::supported_order(d, p) || mesh->d != d || cfg->pde != DGX_PDE_EULER ||
+    const bool visc = cfg->pde == DGX_PDE_NAVIER_STOKES;
+    if (visc && !(cfg->mu >= 0.0 && cfg->Pr > 0.0 && std::isfinite(cfg->mu) && std::isfinite(cfg->Pr)))
+        return DGX_ERR_INVALID_ARG;
+    if (!dgx::supported_order(d, p) || mesh->d != d || (cfg->pde != DGX_PDE_EULER && !visc) ||
         (cfg->num_flux != DGX_FLUX_LAX_FRIEDRICHS && cfg->num_flux != DGX_FLUX_ROE_PIKE) ||
         (cfg->precision != DGX_PREC_FP64 && cfg->precision != DGX_PREC_MIXED))
         return DGX_ERR_INVALID_ARG;
@@ -324,7 +339,13 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     c->precision = cfg->precision;
     c->E = E;
     c->F = F;
-    c->hex3 = (d == 3 && p == 3);
+    c->visc = visc;
+    if (visc) {
+        c->mu = cfg->mu;
+        c->Pr = cfg->Pr;
+        c->eta = 1.01 * nface;   // PENALTY_SCALING * NFMAX (compute_face_rlhs_dg_T.c:276-296)
+    }
+    c->hex3 = (d == 3 && p == 3 && !visc);
 
     std::vector<int32_t> slot_elem;
     if (c->hex3) {
@@ -417,7 +438,38 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         }
     }
 
-    c->nmax = c->hex3 ? Ep : dgx::stage_grid(d, p, E);
+    // NS lift trace factors lam_f(m) = w_f(m) sum_k l_f(k)^2 / (w J)(node(k, m)) (the trace on face f of
+    // the lifting of a unit face datum at m)
+    const int nf = (d == 2) ? n1 : n1 * n1;
+    std::vector<double> lam;
+    if (visc) {
+        lam.assign((size_t)Ep * nface * nf, 0.0);
+        for (int64_t s = 0; s < Ep; ++s) {
+            if (slot_elem[s] < 0)
+                continue;
+            for (int f = 0; f < nface; ++f) {
+                const int dn = f / 2;
+                const double* l = (f & 1) ? o.lp : o.lm;
+                for (int m = 0; m < nf; ++m) {
+                    const int m0 = m % n1, m1 = m / n1;
+                    const double wf = (d == 2) ? o.w[m] : o.w[m0] * o.w[m1];
+                    double acc = 0.0;
+                    for (int k = 0; k < n1; ++k) {
+                        int node;
+                        if (d == 2)
+                            node = (dn == 0) ? k + n1 * m : m + n1 * k;
+                        else
+                            node = (dn == 0) ? k + n1 * (m0 + n1 * m1)
+                                             : (dn == 1) ? m0 + n1 * (k + n1 * m1) : m0 + n1 * (m1 + n1 * k);
+                        acc += l[k] * l[k] * imass[s * ns + node];
+                    }
+                    lam[((size_t)s * nface + f) * nf + m] = wf * acc;
+                }
+            }
+        }
+    }
+
+    c->nmax = c->hex3 ? Ep : (visc ? dgx::ns_grid(d, p, E) : dgx::stage_grid(d, p, E));
     const size_t sbytes = sizeof(double) * (size_t)Ep * c->nv * ns;
     const size_t tbytes = sizeof(double) * (size_t)Ep * nface * c->nv * c->nf;
     dgx_status st = DGX_OK;
@@ -443,6 +495,10 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         step(hipMalloc(&c->face_list, flist.size()), "hipMalloc(face_list)");
         step(hipMalloc(&c->face_count, fcount.size()), "hipMalloc(face_count)");
     }
+    if (visc) {
+        step(hipMalloc(&c->gtr, sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "hipMalloc(gtr)");
+        step(hipMalloc(&c->lam, sizeof(double) * lam.size()), "hipMalloc(lam)");
+    }
     step(hipMalloc(&c->blockmax, sizeof(double) * c->nmax), "hipMalloc(blockmax)");
     step(hipMalloc(&c->dmax, sizeof(double)), "hipMalloc(dmax)");
     step(hipMalloc(&c->bad_elem, sizeof(unsigned long long)), "hipMalloc(bad)");
@@ -465,6 +521,10 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         step(hipMemset(c->tr[0], 0, tbytes), "memset tr0");
         step(hipMemset(c->tr[1], 0, tbytes), "memset tr1");
         step(hipMemset(c->blockmax, 0, sizeof(double) * c->nmax), "memset blockmax");
+        if (visc) {
+            step(hipMemcpy(c->lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice), "copy lam");
+            step(hipMemset(c->gtr, 0, sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "memset gtr");
+        }
         step(dgx::upload_ops(p, ops), "upload ops");
     }
     if (st == DGX_OK)
@@ -658,6 +718,8 @@ static dgx_status lin_apply(dgx_ctx* c, bool adjoint, int apply_minv, int layout
     if (!in || !out || (layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
         (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
         return fail(c, DGX_ERR_INVALID_ARG, std::string(who) + ": bad arguments");
+    if (c->visc)
+        return fail(c, DGX_ERR_INVALID_ARG, std::string(who) + ": the linearized residual is built for Euler");
     const size_t sbytes = sizeof(double) * (size_t)c->Ep * c->nv * c->ns;
     const size_t tbytes = sizeof(double) * (size_t)c->Ep * c->nface * c->nv * c->nf;
     if (!c->lin_x) {

@@ -23,6 +23,8 @@ hipError_t launch_reduce_max(const double* bm, int64_t nb, double* out, unsigned
 hipError_t launch_lin(int d, int p, int flux, bool adjoint, const LinArgs& args, hipStream_t st);
 hipError_t launch_scale_by_imass(const double* x, const double* imass, double* y, int64_t E, int nv, int ns,
                                  hipStream_t st);
+hipError_t launch_ns_stage(int d, int p, int flux, const StageArgs& args, double* gtr, hipStream_t st);
+int64_t ns_grid(int d, int p, int64_t n_elem);
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
 
 }  // namespace dgx

@@ -265,7 +265,57 @@ struct StageArgs {
     int stage;              // 0,1,2: SSP-RK3 stage; -1: residual only
     int apply_minv;         // residual mode: 1 = M^-1 applied
     int positivity;
+    // Navier-Stokes only
+    const double* gtr;      // [E][NFACE][D*NV][NF] face traces of the volume gradient of the current state
+    const double* lam;      // [E][NFACE][NF] lift trace factor w_f(m) sum_k l_f(k)^2 / (w(k) w_f(m) J)(k, m)
+    double mu, Pr, eta;     // viscosity, Prandtl number, BR2 penalty
 };
+
+// Second-order (viscous) part of the Navier-Stokes flux at one point, negated as the reference stores
+// it (flux_navier_stokes_T.c:1005-1040): f2[i + D*eq] = -F_v(i, eq).  g[v + NV*j] = d(var v)/dx_j.
+// tau (compute_tau :778-799), velocity gradient (compute_duvw :708-720), internal-energy gradient
+// (compute_dTs :904-925), energy flux gamma/Pr mu grad e + u.tau.
+template <int D>
+__device__ inline void viscous_flux(const double* w, const double* g, double mu, double Pr, double* f2)
+{
+    constexpr int NV = D + 2;
+    const double rho_inv = rcp_nr(w[0]);
+    double uvw[D], duvw[D][D], dTs[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+        uvw[k] = rho_inv * w[1 + k];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            duvw[i][j] = rho_inv * (g[(1 + i) + NV * j] - g[NV * j] * uvw[i]);
+    double divV = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+        divV += duvw[k][k];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        const double dE_o_rho = rho_inv * rho_inv * (g[(D + 1) + NV * j] * w[0] - w[D + 1] * g[NV * j]);
+        double dV2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            dV2 += uvw[k] * duvw[k][j];
+        dTs[j] = dE_o_rho - dV2;
+    }
+    const double cE = kGamma / Pr * mu;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        f2[i] = 0.0;
+        double ut = 0.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            const double tau = (i == m) ? mu * 2.0 * (duvw[i][i] - divV * (1.0 / 3.0)) : mu * (duvw[m][i] + duvw[i][m]);
+            f2[i + D * (1 + m)] = -tau;
+            ut += uvw[m] * tau;
+        }
+        f2[i + D * (D + 1)] = -(cE * dTs[i] + ut);
+    }
+}
 
 // Arguments of the HEX p=3 group kernel; arrays are in slot order (slot = group*8 + wave).
 struct Hex3Args {

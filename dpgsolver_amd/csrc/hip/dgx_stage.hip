@@ -146,167 +146,19 @@ __device__ inline void write_traces(const double* s, double* tr, int64_t e, int 
     }
 }
 
-// ---------------------------------------------------------------------------------------------------
-// fused stage kernel
-// ---------------------------------------------------------------------------------------------------
-
-template <int D, int P, int FLUX>
-__global__ __launch_bounds__(256) void stage_kernel(StageArgs a)
+// Block max of |rhs|, then residual output (stage < 0) or the SSP-RK3 combination, the positivity
+// limiter, the state write and the face traces of the updated state.  Every thread of the block calls
+// it (it holds barriers); su is the element's [NV][NS] tile (free to overwrite), rhs the node's residual.
+template <int D, int P>
+__device__ inline void stage_tail(const StageArgs& a, int64_t e, int n, int el, bool in_tile, bool valid,
+                                  double* su, double* s_red, double* s_wmax, const double* rhs, const double* uu,
+                                  double lmax)
 {
     using S = Shape<D, P>;
-    constexpr int NS = S::NS, NV = S::NV, NF = S::NF, NFACE = S::NFACE, EPB = S::EPB, N1 = S::N1;
-    constexpr int NSP = S::NSP, DD = D * D;
-    constexpr int REG_A = NV * NS + DD * NS;
-    constexpr int REG_B = NV * D * NS;
-    constexpr int REGION = REG_A > REG_B ? REG_A : REG_B;
-    constexpr int NRED = (NV + 1) * NSP;
-    __shared__ double s_reg[EPB * REGION];
-    __shared__ double s_nnf[EPB * NFACE * NV * NF];
-    __shared__ double s_red[EPB * NRED];
-    __shared__ double s_wmax[4];
-
+    constexpr int NS = S::NS, NV = S::NV;
     const Ops1D& op = c_ops[P];
     const int tid = threadIdx.x;
-    const int el = tid / NS;
-    const int n = tid - el * NS;
-    const bool in_tile = el < EPB;
     const int64_t E = a.n_elem;
-    const int64_t e = (int64_t)blockIdx.x * EPB + el;
-    const bool valid = in_tile && e < E;
-
-    double* su = s_reg + (in_tile ? el : 0) * REGION;   // [NV][NS]
-    double* sm = su + NV * NS;                          // [DD][NS]
-    double* sg = su;                                    // [NV][D][NS] (after the face phase)
-    double* snnf = s_nnf + (in_tile ? el : 0) * NFACE * NV * NF;
-
-    double uu[NV], mm[DD];
-    if (valid) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            uu[v] = a.u[(v * E + e) * NS + n];
-            su[v * NS + n] = uu[v];
-        }
-#pragma unroll
-        for (int c = 0; c < DD; ++c) {
-            mm[c] = a.met[(c * E + e) * NS + n];
-            sm[c * NS + n] = mm[c];
-        }
-    }
-    __syncthreads();
-
-    // ---- face phase: numerical flux at every face point of the element -------------------------
-    if (valid) {
-        for (int q = n; q < NFACE * NF; q += NS) {
-            const int f = q / NF, m = q % NF, dn = f >> 1, side = f & 1;
-            const double* l = op.l[side];
-            double wo[NV], nv[D];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                double acc = 0.0;
-#pragma unroll
-                for (int k = 0; k < N1; ++k)
-                    acc += l[k] * su[v * NS + face_node<D, P>(dn, k, m)];
-                wo[v] = acc;
-            }
-            double jf = 0.0;
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                double acc = 0.0;
-#pragma unroll
-                for (int k = 0; k < N1; ++k)
-                    acc += l[k] * sm[(i + D * dn) * NS + face_node<D, P>(dn, k, m)];
-                nv[i] = side ? acc : -acc;
-                jf += nv[i] * nv[i];
-            }
-            jf = sqrt(jf);
-            const double jf_inv = 1.0 / jf;
-            const uint32_t rec = a.nbr[e * NFACE + f];
-            const int64_t en = rec & 0x0FFFFFFFu;
-            const int lfn = (rec >> 28) & 7;
-            const bool right = (rec >> 31) != 0;
-            double wn[NV], nf[NV];
-            const double* src = a.tr_in + ((en * NFACE + lfn) * NV) * NF + m;
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-                wn[v] = src[v * NF];
-            double nh[D];
-            if (!right) {
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-                    nh[i] = nv[i] * jf_inv;
-                numflux<D, FLUX>(wo, wn, nh, nf);
-            } else {
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-                    nh[i] = -nv[i] * jf_inv;
-                numflux<D, FLUX>(wn, wo, nh, nf);
-            }
-            const double scale = (right ? -jf : jf) * face_weight<D, P>(op, m);
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-                snnf[(f * NV + v) * NF + m] = nf[v] * scale;
-        }
-    }
-    __syncthreads();
-
-    // ---- volume: weighted reference flux g(eq, j) = W sum_i m(i,j) F_i(eq) -----------------------
-    int idx[3];
-    node_index<D, P>(n, idx);
-    if (valid) {
-        double W = op.w[idx[0]] * op.w[idx[1]];
-        if constexpr (D == 3)
-            W *= op.w[idx[2]];
-#pragma unroll
-        for (int j = 0; j < D; ++j) {
-            double dirv[D];
-#pragma unroll
-            for (int i = 0; i < D; ++i)
-                dirv[i] = mm[i + D * j] * W;
-            double fj[NV];
-            euler_flux_n<D>(uu, dirv, fj);
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-                sg[(v * D + j) * NS + n] = fj[v];
-        }
-    }
-    __syncthreads();
-
-    // ---- weak divergence + lift + inverse mass ---------------------------------------------------
-    double rhs[NV];
-    double lmax = 0.0;
-    if (valid) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            double r = 0.0;
-#pragma unroll
-            for (int j = 0; j < D; ++j) {
-                const int st = stride_of<P>(j);
-                const int base = n - idx[j] * st;
-                const double* g = sg + (v * D + j) * NS + base;
-#pragma unroll
-                for (int q = 0; q < N1; ++q)
-                    r += op.D[q * N1 + idx[j]] * g[q * st];
-            }
-#pragma unroll
-            for (int f = 0; f < NFACE; ++f) {
-                const int dn = f >> 1, side = f & 1;
-                const int m = tangential_index<D, P>(dn, idx);
-                r -= op.l[side][idx[dn]] * snnf[(f * NV + v) * NF + m];
-            }
-            rhs[v] = r;
-        }
-        const bool minv = (a.stage >= 0) || a.apply_minv;
-        if (minv) {
-            const double im = a.imass[e * NS + n];
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-                rhs[v] *= im;
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-            lmax = nan_max(lmax, fabs(rhs[v]));
-    }
-
     // block max of |rhs| -> one value per block (reduced by a second kernel when requested)
     {
         double m = lmax;
@@ -472,6 +324,170 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs a)
             a.u[(v * E + e) * NS + n] = un[v];
     }
     write_traces<D, P>(su, a.tr_out, e, n, valid);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// fused stage kernel
+// ---------------------------------------------------------------------------------------------------
+
+template <int D, int P, int FLUX>
+__global__ __launch_bounds__(256) void stage_kernel(StageArgs a)
+{
+    using S = Shape<D, P>;
+    constexpr int NS = S::NS, NV = S::NV, NF = S::NF, NFACE = S::NFACE, EPB = S::EPB, N1 = S::N1;
+    constexpr int NSP = S::NSP, DD = D * D;
+    constexpr int REG_A = NV * NS + DD * NS;
+    constexpr int REG_B = NV * D * NS;
+    constexpr int REGION = REG_A > REG_B ? REG_A : REG_B;
+    constexpr int NRED = (NV + 1) * NSP;
+    __shared__ double s_reg[EPB * REGION];
+    __shared__ double s_nnf[EPB * NFACE * NV * NF];
+    __shared__ double s_red[EPB * NRED];
+    __shared__ double s_wmax[4];
+
+    const Ops1D& op = c_ops[P];
+    const int tid = threadIdx.x;
+    const int el = tid / NS;
+    const int n = tid - el * NS;
+    const bool in_tile = el < EPB;
+    const int64_t E = a.n_elem;
+    const int64_t e = (int64_t)blockIdx.x * EPB + el;
+    const bool valid = in_tile && e < E;
+
+    double* su = s_reg + (in_tile ? el : 0) * REGION;   // [NV][NS]
+    double* sm = su + NV * NS;                          // [DD][NS]
+    double* sg = su;                                    // [NV][D][NS] (after the face phase)
+    double* snnf = s_nnf + (in_tile ? el : 0) * NFACE * NV * NF;
+
+    double uu[NV], mm[DD];
+    if (valid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            uu[v] = a.u[(v * E + e) * NS + n];
+            su[v * NS + n] = uu[v];
+        }
+#pragma unroll
+        for (int c = 0; c < DD; ++c) {
+            mm[c] = a.met[(c * E + e) * NS + n];
+            sm[c * NS + n] = mm[c];
+        }
+    }
+    __syncthreads();
+
+    // ---- face phase: numerical flux at every face point of the element -------------------------
+    if (valid) {
+        for (int q = n; q < NFACE * NF; q += NS) {
+            const int f = q / NF, m = q % NF, dn = f >> 1, side = f & 1;
+            const double* l = op.l[side];
+            double wo[NV], nv[D];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < N1; ++k)
+                    acc += l[k] * su[v * NS + face_node<D, P>(dn, k, m)];
+                wo[v] = acc;
+            }
+            double jf = 0.0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < N1; ++k)
+                    acc += l[k] * sm[(i + D * dn) * NS + face_node<D, P>(dn, k, m)];
+                nv[i] = side ? acc : -acc;
+                jf += nv[i] * nv[i];
+            }
+            jf = sqrt(jf);
+            const double jf_inv = 1.0 / jf;
+            const uint32_t rec = a.nbr[e * NFACE + f];
+            const int64_t en = rec & 0x0FFFFFFFu;
+            const int lfn = (rec >> 28) & 7;
+            const bool right = (rec >> 31) != 0;
+            double wn[NV], nf[NV];
+            const double* src = a.tr_in + ((en * NFACE + lfn) * NV) * NF + m;
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                wn[v] = src[v * NF];
+            double nh[D];
+            if (!right) {
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+                    nh[i] = nv[i] * jf_inv;
+                numflux<D, FLUX>(wo, wn, nh, nf);
+            } else {
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+                    nh[i] = -nv[i] * jf_inv;
+                numflux<D, FLUX>(wn, wo, nh, nf);
+            }
+            const double scale = (right ? -jf : jf) * face_weight<D, P>(op, m);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                snnf[(f * NV + v) * NF + m] = nf[v] * scale;
+        }
+    }
+    __syncthreads();
+
+    // ---- volume: weighted reference flux g(eq, j) = W sum_i m(i,j) F_i(eq) -----------------------
+    int idx[3];
+    node_index<D, P>(n, idx);
+    if (valid) {
+        double W = op.w[idx[0]] * op.w[idx[1]];
+        if constexpr (D == 3)
+            W *= op.w[idx[2]];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            double dirv[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+                dirv[i] = mm[i + D * j] * W;
+            double fj[NV];
+            euler_flux_n<D>(uu, dirv, fj);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                sg[(v * D + j) * NS + n] = fj[v];
+        }
+    }
+    __syncthreads();
+
+    // ---- weak divergence + lift + inverse mass ---------------------------------------------------
+    double rhs[NV];
+    double lmax = 0.0;
+    if (valid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double r = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const int st = stride_of<P>(j);
+                const int base = n - idx[j] * st;
+                const double* g = sg + (v * D + j) * NS + base;
+#pragma unroll
+                for (int q = 0; q < N1; ++q)
+                    r += op.D[q * N1 + idx[j]] * g[q * st];
+            }
+#pragma unroll
+            for (int f = 0; f < NFACE; ++f) {
+                const int dn = f >> 1, side = f & 1;
+                const int m = tangential_index<D, P>(dn, idx);
+                r -= op.l[side][idx[dn]] * snnf[(f * NV + v) * NF + m];
+            }
+            rhs[v] = r;
+        }
+        const bool minv = (a.stage >= 0) || a.apply_minv;
+        if (minv) {
+            const double im = a.imass[e * NS + n];
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                rhs[v] *= im;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            lmax = nan_max(lmax, fabs(rhs[v]));
+    }
+
+    stage_tail<D, P>(a, e, n, el, in_tile, valid, su, s_red, s_wmax, rhs, uu, lmax);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -690,3 +706,4 @@ hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, h
 
 #include "dgx_hex3.inc"
 #include "dgx_lin.inc"
+#include "dgx_ns.inc"

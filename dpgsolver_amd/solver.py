@@ -24,11 +24,12 @@ class Solver:
     """Explicit DG (Euler, QUAD/HEX, GL/GL collocated) on one GPU.
 
     mesh, geo: dpgsolver_amd.problem.Mesh / Geometry; p: solution order; num_flux: "lf" | "roe";
-    positivity: apply enforce_positivity_highorder after every stage (the reference does for Euler,
-    test_case.c:63-79).
+    positivity: apply enforce_positivity_highorder after every stage (the reference does for Euler and
+    Navier-Stokes, test_case.c:63-79); viscous: None (Euler) or (mu, Pr) for Navier-Stokes with the
+    BR2-stable viscous flux.
     """
 
-    def __init__(self, mesh, geo, p, num_flux="roe", positivity=True, precision="fp64"):
+    def __init__(self, mesh, geo, p, num_flux="roe", positivity=True, precision="fp64", viscous=None):
         self._lib = _ffi.load()
         self.d, self.p = mesh.d, p
         self.nv = mesh.d + 2
@@ -39,6 +40,9 @@ class Solver:
         cfg.num_flux = NUM_FLUX[num_flux] if isinstance(num_flux, str) else int(num_flux)
         cfg.positivity = 1 if positivity else 0
         cfg.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
+        if viscous is not None:
+            cfg.pde = _ffi.PDE_NAVIER_STOKES
+            cfg.mu, cfg.Pr = float(viscous[0]), float(viscous[1])
         self._mesh_c = mesh.c_struct()
         self._geo_c = geo.c_struct()
         self._keep = (mesh, geo)

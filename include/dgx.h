@@ -33,7 +33,7 @@ typedef enum dgx_status {
     DGX_ERR_NO_DEVICE = 6         /* no GPU or the device code is missing */
 } dgx_status;
 
-enum { DGX_PDE_EULER = 0 };
+enum { DGX_PDE_EULER = 0, DGX_PDE_NAVIER_STOKES = 1 };   /* test_case.c pde_index: euler, navier_stokes */
 enum { DGX_FLUX_LAX_FRIEDRICHS = 0, DGX_FLUX_ROE_PIKE = 1 };   /* numerical_flux_euler_T.c:60-327, 470-778 */
 enum { DGX_PREC_FP64 = 0, DGX_PREC_MIXED = 1 };
 enum { DGX_LAYOUT_REF = 0, DGX_LAYOUT_SOA = 1 };               /* SOA = device layout u[v][e][n] */
@@ -42,10 +42,16 @@ enum { DGX_PTR_HOST = 0, DGX_PTR_DEVICE = 1 };
 typedef struct dgx_config {
     int32_t d;            /* 2 (QUAD) or 3 (HEX) */
     int32_t p;            /* solution order; GL solution nodes = GL cubature, p+1 per direction */
-    int32_t pde;          /* DGX_PDE_EULER */
+    int32_t pde;          /* DGX_PDE_EULER or DGX_PDE_NAVIER_STOKES */
     int32_t num_flux;     /* DGX_FLUX_* */
     int32_t positivity;   /* 1: enforce_positivity_highorder after every RK stage (solve.c:187-233) */
     int32_t precision;    /* DGX_PREC_* used by dgx_step / dgx_rk_stage */
+    /* Navier-Stokes only (ignored for Euler): constant dynamic viscosity "mu" and Prandtl number "Pr"
+     * of the reference's input file (solution_navier_stokes_T.c:221-240, flux_navier_stokes_T.c:461-483);
+     * the viscous flux is BR2-stable with the central numerical solution and flux, penalty 1.01 x faces
+     * per element (compute_grad_coef_dg_T.c, compute_face_rlhs_dg_T.c:276-296) */
+    double mu;
+    double Pr;
 } dgx_config;
 
 typedef struct dgx_mesh {

@@ -24,6 +24,8 @@ struct orc_ctx {
     /* dense element operators (row-major) */
     double* cv0_vs_vc;          /* Nc x Ns */
     double* tw1_vt_vc[3];       /* Ns x Nc */
+    double* cv1_vs_vc[3];       /* Nc x Ns: derivative along xi_j at the volume cubature nodes */
+    double* tw0_vt_vc;          /* Ns x Nc: cv0_vs_vc^T diag(w_vc) */
     double* cv0_vs_fc[6];       /* Nf x Ns */
     double* tw0_vt_fc[6];       /* Ns x Nf */
     double* ccSB0;              /* Ns x Ns */
@@ -36,6 +38,10 @@ struct orc_ctx {
     double* jdet_fc;            /* [F][Nf] */
     double* m_inv;              /* [E][Ns*Ns], built on first use after the geometry is set */
     int minv_ready;
+    /* Navier-Stokes (constant viscosity) with the BR2-stable second-order flux when ns != 0 */
+    int ns;
+    double mu, Pr;
+    double eta;                 /* BR2 penalty override (> 0), else 1.01 x NFMAX */
 };
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -555,6 +561,15 @@ orc_ctx* orc_create(int d, int p, int p_g, int64_t n_elem, int64_t n_face, const
         A[j] = tw1;
         c->tw1_vt_vc[j] = (double*)malloc(sizeof(double) * Ns * Nc);
         orc_kron(d, ms, ms, A, c->tw1_vt_vc[j]);
+        const double* B[3] = { cv0, cv0, cv0 };
+        B[j] = cv1;
+        c->cv1_vs_vc[j] = (double*)malloc(sizeof(double) * Nc * Ns);
+        orc_kron(d, ms, ms, B, c->cv1_vs_vc[j]);
+    }
+    {
+        const double* A[3] = { tw0, tw0, tw0 };
+        c->tw0_vt_vc = (double*)malloc(sizeof(double) * Ns * Nc);
+        orc_kron(d, ms, ms, A, c->tw0_vt_vc);
     }
     for (int lf = 0; lf < 2 * d; ++lf) {
         const int dn = lf / 2;
@@ -617,12 +632,15 @@ void orc_destroy(orc_ctx* c)
         return;
     free(c->face_elem); free(c->face_lf);
     free(c->cv0_vs_vc);
-    for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < 3; ++j) {
         free(c->tw1_vt_vc[j]);
+        free(c->cv1_vs_vc[j]);
+    }
     for (int lf = 0; lf < 6; ++lf) {
         free(c->cv0_vs_fc[lf]);
         free(c->tw0_vt_fc[lf]);
     }
+    free(c->tw0_vt_vc);
     free(c->ccSB0); free(c->ccBS0); free(c->w_vc);
     free(c->metrics_vc); free(c->jdet_vc); free(c->normals_fc); free(c->jdet_fc); free(c->m_inv);
     free(c);
@@ -668,6 +686,25 @@ void orc_set_geometry(orc_ctx* c, const double* metrics_vc, const double* jdet_v
     memcpy(c->normals_fc, normals_fc, sizeof(double) * (size_t)c->n_face * c->Nf * d);
     memcpy(c->jdet_fc, jdet_fc, sizeof(double) * (size_t)c->n_face * c->Nf);
     c->minv_ready = 0;
+}
+
+void orc_gradient(orc_ctx* c, const double* u, double* grad_v, double* grad)
+{
+    double* grad_f = (double*)malloc(sizeof(double) * (size_t)c->Ns * (c->d + 2) * c->d * 2 * c->n_face);
+    gradients_d(c, u, grad_v, grad_f, grad);
+    free(grad_f);
+}
+
+void orc_set_viscous(orc_ctx* c, double mu, double Pr)
+{
+    c->ns = 1;
+    c->mu = mu;
+    c->Pr = Pr;
+}
+
+void orc_set_penalty(orc_ctx* c, double eta)
+{
+    c->eta = eta;
 }
 
 int orc_operator(const orc_ctx* c, const char* name, double* out)

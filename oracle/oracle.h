@@ -54,6 +54,17 @@ void orc_destroy(orc_ctx* c);
 /* Geometry factors computed by the oracle, in the dgx_geometry layouts. */
 void orc_geometry(const orc_ctx* c, double* metrics_vc, double* jdet_vc, double* normals_fc, double* jdet_fc);
 
+/* Switch the residual to Navier-Stokes: constant viscosity mu, Prandtl number Pr, BR2-stable viscous
+ * flux with the central numerical solution and flux (penalty 1.01 x faces per element). */
+void orc_set_viscous(orc_ctx* c, double mu, double Pr);
+
+/* Test hook: BR2 penalty used instead of 1.01 x NFMAX (NFMAX = 4 for QUAD, 6 for HEX). */
+void orc_set_penalty(orc_ctx* c, double eta);
+
+/* Weak gradients of the Navier-Stokes path at state u, each [E][d][nv][Ns]: grad_v (volume term) and
+ * grad (volume term plus the liftings of all faces of the element). */
+void orc_gradient(orc_ctx* c, const double* u, double* grad_v, double* grad);
+
 /* Replace the oracle's geometry by given factors (same layouts) and rebuild the mass matrices: the
  * residual path is then compared on exactly the geometry the GPU path consumes. */
 void orc_set_geometry(orc_ctx* c, const double* metrics_vc, const double* jdet_vc, const double* normals_fc,

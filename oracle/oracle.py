@@ -27,6 +27,9 @@ _SIGS = {
     "orc_destroy": (None, [ctypes.c_void_p]),
     "orc_geometry": (None, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "orc_set_geometry": (None, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "orc_set_viscous": (None, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "orc_gradient": (None, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "orc_set_penalty": (None, [ctypes.c_void_p, ctypes.c_double]),
     "orc_operator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _dp]),
     "orc_residual": (ctypes.c_double, [ctypes.c_void_p, _dp, _dp, ctypes.c_int]),
     "orc_jvp_cs": (None, [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp, ctypes.c_int]),
@@ -119,7 +122,7 @@ def numflux_point(d, num_flux, ul, ur, n):
 class Oracle:
     """Dense CPU restatement of the explicit residual path on one mesh (see oracle.h)."""
 
-    def __init__(self, mesh, p, num_flux=0, positivity=True, geometry=None):
+    def __init__(self, mesh, p, num_flux=0, positivity=True, geometry=None, viscous=None):
         lib = load()
         self.d, self.p = mesh.d, p
         self.n_elem, self.n_face = mesh.n_elem, mesh.n_face
@@ -135,6 +138,9 @@ class Oracle:
             raise ValueError("orc_create failed")
         if geometry is not None:
             self.set_geometry(geometry)
+        if viscous is not None:
+            mu, pr = viscous
+            lib.orc_set_viscous(self._ptr, float(mu), float(pr))
 
     def __del__(self):
         if getattr(self, "_ptr", None):
@@ -167,6 +173,18 @@ class Oracle:
         rhs = np.empty_like(u)
         mx = load().orc_residual(self._ptr, _d(u), _d(rhs), int(bool(apply_minv)))
         return rhs, mx
+
+    def set_penalty(self, eta):
+        """Test hook: BR2 penalty eta instead of 1.01 x faces per element."""
+        load().orc_set_penalty(self._ptr, float(eta))
+
+    def gradient(self, u):
+        """(grad_v, grad), each [E][d][nv][Ns]: weak volume gradient and the lifted gradient."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        gv = np.empty((self.n_elem, self.d, self.nv, self.Ns))
+        g = np.empty_like(gv)
+        load().orc_gradient(self._ptr, _d(u), _d(gv), _d(g))
+        return gv, g
 
     def jvp_cs(self, u, w, h=1e-30, apply_minv=False):
         u = np.ascontiguousarray(u, dtype=np.float64)

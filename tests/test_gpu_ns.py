@@ -1,0 +1,144 @@
+"""GPU Navier-Stokes (BR2-stable viscous flux) against the oracle, through the C-ABI.
+
+Tolerances as for Euler: relative L2 <= 1e-12 and max-norm <= 1e-11 of the largest entry.  The viscosity
+is raised (mu = 0.05) on the small parity meshes so that the viscous terms are a visible part of the
+residual; the full-size case uses the Taylor-Green set-up at Re = 1600 (mu = 1/1600), Pr = 0.71.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import dpgsolver_amd as dg
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TOL_MAX = 1e-11
+
+
+def _require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    emax = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    assert emax < TOL_MAX, f"max-norm relative error {emax:.3e}"
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def case(d, p, n, seed=3, scale=1e-2):
+    L = 2 * math.pi
+    if d == 3:
+        mesh = dg.structured_mesh(3, (n, n + 1, n), (0, 0, 0), (L, L, L), p_g=p, amp=0.05, seed=seed)
+        u = dg.ic_taylor_green(mesh, p, 0.3)
+    else:
+        mesh = dg.structured_mesh(2, (n, n + 1), (-5, -5), (5, 5), p_g=p, amp=0.04, seed=seed)
+        u = dg.ic_isentropic_vortex(mesh, p)
+    geo = dg.geometry(mesh, p)
+    return mesh, geo, u + scale * dg.randn(seed + 100, u.shape)
+
+
+@pytest.mark.parametrize("d,p,n,num_flux", [
+    (2, 1, 5, "lf"), (2, 2, 4, "roe"), (2, 3, 4, "roe"), (2, 4, 3, "lf"),
+    (3, 1, 3, "roe"), (3, 2, 3, "lf"), (3, 3, 3, "roe"), (3, 4, 2, "roe"),
+])
+def test_ns_residual(d, p, n, num_flux):
+    _require_gpu()
+    print(f"case: ns residual d={d} p={p} n={n} flux={num_flux}", flush=True)
+    mesh, geo, u = case(d, p, n)
+    visc = (0.05, 0.71)
+    orc = Oracle(mesh, p, num_flux=0 if num_flux == "lf" else 1, geometry=geo, viscous=visc)
+    s = dg.Solver(mesh, geo, p, num_flux=num_flux, viscous=visc)
+    s.set_state(u)
+    for minv in (True, False):
+        r_gpu, mx_gpu = s.compute_rhs(apply_minv=minv)
+        r_ref, mx_ref = orc.residual(u, apply_minv=minv)
+        assert rel_err(r_gpu, r_ref) < TOL, minv
+        if minv:
+            assert abs(mx_gpu - mx_ref) <= 1e-11 * mx_ref
+    # the viscous part is a visible share of the residual
+    r_euler, _ = Oracle(mesh, p, num_flux=0 if num_flux == "lf" else 1, geometry=geo).residual(u)
+    r_ref, _ = orc.residual(u)
+    assert np.linalg.norm(r_ref - r_euler) > 1e-3 * np.linalg.norm(r_ref)
+    s.close()
+
+
+def test_ns_steps_vs_oracle():
+    """5 SSP-RK3 steps (positivity on) of 3-D NS, p=3, through dgx_step vs the oracle's stages."""
+    _require_gpu()
+    print("case: ns steps", flush=True)
+    mesh, geo, u0 = case(3, 3, 3, scale=1e-3)
+    visc = (0.05, 0.71)
+    dt = 0.5 * dg.dt_cfl(mesh, 3, u0)
+    orc = Oracle(mesh, 3, num_flux=1, geometry=geo, viscous=visc)
+    u_ref, up = u0.copy(), np.zeros_like(u0)
+    orc.step(dt, u_ref, up, nsteps=5)
+    s = dg.Solver(mesh, geo, 3, num_flux="roe", viscous=visc)
+    s.set_state(u0)
+    s.step(dt, 5)
+    assert rel_err(s.get_state(), u_ref) < TOL
+    s.close()
+
+
+def test_ns_taylor_green_re1600():
+    """Taylor-Green at Re = 1600, M = 0.1, Pr = 0.71, p = 4 on a straight periodic mesh: residual parity
+    at 6^3 against the oracle; at 48^3 (the full problem, 110,592 elements) free-stream preservation,
+    discrete conservation and a decaying kinetic energy over 10 steps."""
+    _require_gpu()
+    import torch
+    print("case: ns tgv 6^3 parity", flush=True)
+    visc = (1.0 / 1600.0, 0.71)
+    L = 2 * math.pi
+    mesh = dg.structured_mesh(3, (6, 6, 6), (0, 0, 0), (L, L, L), p_g=1)
+    geo = dg.geometry(mesh, 4)
+    u = dg.ic_taylor_green(mesh, 4, 0.1)
+    orc = Oracle(mesh, 4, num_flux=1, geometry=geo, viscous=visc)
+    s = dg.Solver(mesh, geo, 4, num_flux="roe", viscous=visc)
+    s.set_state(u)
+    # At M = 0.1 the energy (~180) and pressure (~71) are ~500 times the residual (|r| ~ 0.37), which
+    # the flux differences cancel down to: the residual of this state is ill-conditioned, a relative
+    # perturbation of 1e-16 of the input moves the oracle's own residual by ~1.6e-11 (relative L2).
+    # The bound is that round-off floor, measured here, times 4 (and never below 1e-12).
+    rng = np.random.default_rng(0)
+    u_eps = u * (1 + 1.1e-16 * rng.standard_normal(u.shape))
+    for minv in (False, True):
+        a, b = s.compute_rhs(apply_minv=minv)[0], orc.residual(u, apply_minv=minv)[0]
+        b_eps = orc.residual(u_eps, apply_minv=minv)[0]
+        floor = float(np.linalg.norm(b_eps - b) / np.linalg.norm(b))
+        e2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        emax = float(np.abs(a - b).max() / np.abs(b).max())
+        print(f"tgv 6^3 p=4 minv={minv}: rel L2 {e2:.3e} (round-off floor {floor:.3e}), rel max {emax:.3e}",
+              flush=True)
+        assert e2 < max(TOL, 4 * floor)
+    s.close()
+
+    print("case: ns tgv 48^3", flush=True)
+    mesh = dg.structured_mesh(3, (48, 48, 48), (0, 0, 0), (L, L, L), p_g=1)
+    geo = dg.geometry(mesh, 4)
+    u0 = dg.ic_taylor_green(mesh, 4, 0.1)
+    s = dg.Solver(mesh, geo, 4, num_flux="roe", viscous=visc)
+    const = np.empty_like(u0)
+    const[:, 0], const[:, 1], const[:, 2], const[:, 3], const[:, 4] = 1.0, 0.3, -0.2, 0.1, 100.0
+    s.set_state(torch.from_numpy(const).cuda())
+    r, _ = s.compute_rhs()
+    assert np.abs(r).max() < 1e-10
+    s.set_state(u0)
+    r, _ = s.compute_rhs(apply_minv=False)
+    assert np.abs(r.sum(axis=(0, 2))).max() < 1e-9 * np.abs(r).max() * r.shape[0]
+
+    dt = dg.dt_cfl(mesh, 4, u0)
+
+    def kinetic(uu):
+        return float(np.sum(0.5 * (uu[:, 1] ** 2 + uu[:, 2] ** 2 + uu[:, 3] ** 2) / uu[:, 0]))
+
+    k0 = kinetic(u0)
+    s.step(dt, 10)
+    k1 = kinetic(s.get_state())
+    print(f"kinetic energy (nodal sum) {k0:.6e} -> {k1:.6e} after 10 steps", flush=True)
+    assert k1 < k0 and k1 > 0.99 * k0
+    s.close()
