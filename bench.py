@@ -79,12 +79,82 @@ def cpu_baseline(args, d, p, num_flux):
             "sample": f"{sample}; {stages} stages in {el:.1f} s (oracle, dense per-element operators)"}
 
 
+def precision_form(args, dg, torch, mesh, geo, u0, dt, fp64_solver, fp64_stage_ms):
+    """The precision form (FP32 hi/lo-split contractions, FP32 accumulation) against the FP64 run just
+    timed: stage time ratio, residual error at the initial state, state error after the same steps,
+    and FP64's own round-off floor (residual change under a 1e-16 relative input perturbation)."""
+    dev = torch.device("cuda:0")
+    ut = torch.from_numpy(u0).to(dev)
+    mixed = dg.Solver(mesh, geo, 3, num_flux="roe", precision="mixed")
+    mixed.set_state(ut)
+    if args.warmup > 0:
+        mixed.step(dt, args.warmup)
+    mixed.step(dt, args.steps)
+    mixed_stage_ms = mixed.last_elapsed_ms / (3 * args.steps)
+    u_fp64 = fp64_solver.get_state(torch.empty_like(ut))
+    u_mixed = mixed.get_state(torch.empty_like(ut))
+    mixed.set_state(ut)
+    r_mixed = torch.empty_like(ut)
+    mixed.compute_rhs(precision="mixed", out=r_mixed)
+    mixed.close()
+    fp64_solver.set_state(ut)
+    r64 = torch.empty_like(ut)
+    fp64_solver.compute_rhs(out=r64)
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    fp64_solver.set_state(ut * (1 + 1.1e-16 * torch.randn(ut.shape, generator=gen, dtype=torch.float64).to(dev)))
+    r_eps = torch.empty_like(ut)
+    fp64_solver.compute_rhs(out=r_eps)
+
+    def rel(a, b):
+        return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+    return {"kernel": "dgx::hex3_stage_kernel<1,true>", "speedup_vs_fp64": fp64_stage_ms / mixed_stage_ms,
+            "stage_us": mixed_stage_ms * 1e3, "residual_rel_l2": rel(r_mixed, r64),
+            "state_rel_l2": rel(u_mixed, u_fp64), "state_steps": args.warmup + args.steps,
+            "fp64_roundoff_floor_rel_l2": rel(r_eps, r64), "target_rel_err": 1e-9,
+            "meets_target": rel(r_mixed, r64) <= 1e-9}
+
+
+def kinetic_energy(u):
+    return float((0.5 * (u[:, 1] ** 2 + u[:, 2] ** 2 + u[:, 3] ** 2) / u[:, 0]).sum())
+
+
+def bench_vjp(args, dg, torch):
+    """J^T v of the pre-M^-1 residual at 32^3 HEX p=3 (Taylor-Green M=0.1 + 1e-3 N(0,1) noise), device
+    buffers; one unit = one J^T v over the whole mesh."""
+    mesh, geo, u0 = dg.tgv_3d(32, 3, seed=args.seed)
+    u = u0 + 1e-3 * dg.randn(1234, u0.shape)
+    s = dg.Solver(mesh, geo, 3, num_flux="roe")
+    s.set_state(torch.from_numpy(u).cuda())
+    v = torch.from_numpy(dg.randn(10_000, u.shape)).cuda()
+    out = torch.empty_like(v)
+    for _ in range(max(args.warmup, 1)):
+        s.vjp(v, out=out)
+    ms = []
+    for _ in range(args.steps):
+        s.vjp(v, out=out)
+        ms.append(s.last_elapsed_ms)       # HIP events around upload, traces, adjoint kernel
+    t = sorted(ms)[len(ms) // 2] * 1e-3
+    b_elem = algorithmic_bytes_per_element(3, 3)
+    achieved = b_elem * mesh.n_elem / t / 1e9
+    print(json.dumps({"metric": "J^T v of the residual, 32^3 HEX p=3 Euler", "value": 1.0 / t, "unit": "VJP/s",
+                      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                      "data": "synthetic",
+                      "config": {"workload": "residual adjoint 32^3 curved HEX p=3, TGV M=0.1 + 1e-3 noise, Roe-Pike",
+                                 "elements": mesh.n_elem},
+                      "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                                   "kernel": "dgx::lin_kernel<3,3,1,true> (+ layout and trace kernels)"}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="hex64", choices=["hex64", "hex32", "vortex2d"])
+    ap.add_argument("--config", default="hex64", choices=["hex64", "hex32", "vortex2d", "ns48", "vjp32"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -106,18 +176,29 @@ def main():
         sys.exit(1)
 
     t_setup = time.perf_counter()
+    viscous = None
     if args.config == "vortex2d":
         d, p, num_flux = 2, 2, 0
         mesh, geo, u0 = dg.vortex_2d(32, p)
         workload = "2D Euler QUAD p=2 32^2 periodic, isentropic vortex, Lax-Friedrichs, SSP-RK3"
+    elif args.config == "ns48":
+        d, p, num_flux = 3, 4, 1
+        viscous = (1.0 / 1600.0, 0.71)
+        mesh = dg.structured_mesh(3, (48, 48, 48), (0, 0, 0), (2 * math.pi,) * 3, p_g=1)
+        geo = dg.geometry(mesh, p)
+        u0 = dg.ic_taylor_green(mesh, p, 0.1)
+        workload = ("3D Navier-Stokes HEX p=4 48^3 periodic [0,2pi]^3, Taylor-Green Re=1600 M=0.1 Pr=0.71, "
+                    "Roe-Pike + BR2, SSP-RK3 + positivity")
     else:
         d, p, num_flux = 3, 3, 1
         n = 64 if args.config == "hex64" else 32
+        if args.config == "vjp32":
+            return bench_vjp(args, dg, torch)
         mesh, geo, u0 = dg.tgv_3d(n, p, seed=args.seed)
         workload = (f"3D Euler HEX p=3 {n}^3 curved periodic [0,2pi]^3, Taylor-Green M=0.1, Roe-Pike, "
                     f"SSP-RK3 + positivity")
     dt = dg.dt_cfl(mesh, p, u0, 0.5)
-    solver = dg.Solver(mesh, geo, p, num_flux=num_flux, precision=args.precision)
+    solver = dg.Solver(mesh, geo, p, num_flux=num_flux, precision=args.precision, viscous=viscous)
     solver.set_state(u0)
     t_setup = time.perf_counter() - t_setup
     print(f"[bench] setup {t_setup:.1f} s, E={mesh.n_elem}, dt={dt:.3e}", file=sys.stderr, flush=True)
@@ -146,7 +227,8 @@ def main():
           f"max_rhs {mx:.4e}", file=sys.stderr, flush=True)
 
     out = {
-        "metric": "DOF-stages/s Euler HEX p=3: mixed vs FP64 at <=1e-9 err; % HBM roofline",
+        "metric": ("DOF-stages/s Euler HEX p=3: mixed vs FP64 at <=1e-9 err; % HBM roofline" if not viscous
+                   else "DOF-stages/s Navier-Stokes HEX p=4 (BR2); % HBM roofline"),
         "value": value,
         "unit": "DOF-stages/s",
         "n_gpus": 1,
@@ -162,12 +244,20 @@ def main():
                    "stages_per_step": 3, "precision": args.precision, "dt": dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": (f"dgx::hex3_stage_kernel<{num_flux}>" if (d, p) == (3, 3)
+                     "kernel": (f"dgx::ns_gtrace_kernel<{d},{p}> + dgx::ns_stage_kernel<{d},{p},{num_flux}>"
+                                if viscous else f"dgx::hex3_stage_kernel<{num_flux}>" if (d, p) == (3, 3)
                                 else f"dgx::stage_kernel<{d},{p},{num_flux}>"),
                      "algorithmic_bytes_per_launch": bytes_alg, "bytes_per_element": b_elem,
                      "stage_us": t_stage_s * 1e6},
     }
-    if not args.no_cpu_baseline:
+    if args.config in ("hex64", "hex32") and args.precision == "fp64":
+        out["precision_form"] = precision_form(args, dg, torch, mesh, geo, u0, dt, solver, ms / stages)
+    if viscous:
+        # kinetic-energy history of the run (nodal sum of rho |u|^2 / 2 over the whole mesh)
+        uu = solver.get_state()
+        out["kinetic_energy"] = {"initial": kinetic_energy(u0), "final": kinetic_energy(uu),
+                                 "steps": args.warmup + args.steps, "dt": dt}
+    if not args.no_cpu_baseline and not viscous:
         out["cpu_baseline"] = cpu_baseline(args, d, p, num_flux)
     print(json.dumps(out), flush=True)
 

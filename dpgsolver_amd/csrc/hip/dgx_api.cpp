@@ -116,7 +116,7 @@ dgx_status check_flags(dgx_ctx* c, int64_t* elem)
 }
 
 // Launches one stage (stage 0..2) or the residual (stage -1) reading traces tr[tin].
-hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin)
+hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin, int precision)
 {
     if (c->hex3) {
         dgx::Hex3Args a;
@@ -139,7 +139,7 @@ hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin)
         a.stage = stage;
         a.apply_minv = apply_minv;
         a.positivity = c->positivity;
-        return dgx::launch_hex3_stage(c->flux, a, c->stream);
+        return dgx::launch_hex3_stage(c->flux, precision == DGX_PREC_MIXED, a, c->stream);
     }
     dgx::StageArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -346,6 +346,10 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         c->eta = 1.01 * nface;   // PENALTY_SCALING * NFMAX (compute_face_rlhs_dg_T.c:276-296)
     }
     c->hex3 = (d == 3 && p == 3 && !visc);
+    if (cfg->precision == DGX_PREC_MIXED && !c->hex3) {
+        delete c;
+        return DGX_ERR_INVALID_ARG;   // the precision form is built for Euler HEX p=3
+    }
 
     std::vector<int32_t> slot_elem;
     if (c->hex3) {
@@ -605,11 +609,13 @@ dgx_status dgx_rhs(dgx_ctx* ctx, int precision, int apply_minv, int layout, int 
         return DGX_ERR_INVALID_ARG;
     if (precision != DGX_PREC_FP64 && precision != DGX_PREC_MIXED)
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: bad precision");
+    if (precision == DGX_PREC_MIXED && !ctx->hex3)
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: the precision form is built for Euler HEX p=3");
     if (rhs && ((layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
                 (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE)))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: bad layout");
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(ctx, launch_any(ctx, -1, 0.0, apply_minv ? 1 : 0, ctx->cur));
+    HIP_TRY(ctx, launch_any(ctx, -1, 0.0, apply_minv ? 1 : 0, ctx->cur, precision));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     dgx_status st = read_max(ctx, max_rhs);
     if (st != DGX_OK)
@@ -633,7 +639,7 @@ dgx_status dgx_rk_stage(dgx_ctx* ctx, int stage, double dt, double* max_rhs)
     if (stage < 0 || stage > 2 || !std::isfinite(dt))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rk_stage: bad arguments");
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(ctx, launch_any(ctx, stage, dt, 1, ctx->cur));
+    HIP_TRY(ctx, launch_any(ctx, stage, dt, 1, ctx->cur, ctx->precision));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     ctx->cur ^= 1;
     dgx_status st = read_max(ctx, max_rhs);
@@ -657,7 +663,7 @@ static dgx_status build_graph(dgx_ctx* c, int parity, double dt)
     int cur = parity;
     hipError_t le = hipSuccess;
     for (int s = 0; s < 3 && le == hipSuccess; ++s) {
-        le = launch_any(c, s, dt, 1, cur);
+        le = launch_any(c, s, dt, 1, cur, c->precision);
         cur ^= 1;
     }
     hipError_t ee = hipStreamEndCapture(c->stream, &g);

@@ -17,7 +17,7 @@ hipError_t launch_to_internal(const double* src, int layout, double* dst, const 
                               int64_t Ep, int nv, int ns, hipStream_t st);
 hipError_t launch_from_internal(const double* src, double* dst, int layout, const int32_t* elem_slot, int64_t E,
                                 int64_t Ep, int nv, int ns, hipStream_t st);
-hipError_t launch_hex3_stage(int flux, const Hex3Args& args, hipStream_t st);
+hipError_t launch_hex3_stage(int flux, bool mixed, const Hex3Args& args, hipStream_t st);
 hipError_t launch_hex3_trace(const double* u, double* tr, const int32_t* slot_elem, int64_t n_slots, hipStream_t st);
 hipError_t launch_reduce_max(const double* bm, int64_t nb, double* out, unsigned int* nonfinite, hipStream_t st);
 hipError_t launch_lin(int d, int p, int flux, bool adjoint, const LinArgs& args, hipStream_t st);

@@ -1,0 +1,87 @@
+"""Precision form (DGX_PREC_MIXED) of the HEX p=3 stage kernel, through the C-ABI.
+
+The face traces, the weak derivative and the face lift contract FP32 hi/lo parts of each FP64
+coefficient with FP32 operator entries and FP32 accumulation; everything else (fluxes, Roe
+eigen-decomposition and entropy fix, positivity, metric terms, residual accumulation, RK update) stays
+FP64.  These tests check that the form runs and how far it is from the FP64 path:
+- its residual error against FP64 is that of FP32 rounding of the contraction sums relative to the
+  residual: it falls with the Mach number squared (flux size over residual size);
+- it never touches the FP64 path: FP64 results are bit-identical whether or not a mixed context ran;
+- it is refused where it is not built (other orders, Navier-Stokes).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import dpgsolver_amd as dg
+
+pytestmark = pytest.mark.gpu
+
+
+def _require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def test_mixed_residual_error():
+    """Residual error of the precision form against FP64 on 8^3 Taylor-Green states at M = 0.1 and 0.3.
+    FP32 accumulation leaves errors of ~2^-24 of the contraction inputs (fluxes ~ p ~ 1/(gamma M^2)),
+    which the residual (the flux divergence, O(1)) does not cancel: the relative error grows like
+    1/M^2 and is far above FP64's (measured at 64^3, M = 0.1: 3e-2 against a round-off floor of 2e-10)."""
+    _require_gpu()
+    errs = {}
+    for mach in (0.1, 0.3):
+        print(f"case: precision form 8^3 mach={mach}", flush=True)
+        mesh, geo, u0 = dg.tgv_3d(8, 3, seed=0, mach=mach)
+        u = u0 + 1e-3 * dg.randn(5, u0.shape)
+        a = dg.Solver(mesh, geo, 3, num_flux="roe")
+        b = dg.Solver(mesh, geo, 3, num_flux="roe", precision="mixed")
+        a.set_state(u)
+        b.set_state(u)
+        r64, _ = a.compute_rhs()
+        rmx, _ = b.compute_rhs(precision="mixed")
+        errs[mach] = float(np.linalg.norm(rmx - r64) / np.linalg.norm(r64))
+        print(f"mixed vs fp64 residual rel L2 {errs[mach]:.3e}", flush=True)
+        # the FP64 path is untouched by the mixed context
+        r64b, _ = a.compute_rhs()
+        assert np.array_equal(r64, r64b)
+        a.close()
+        b.close()
+    assert 0 < errs[0.3] < errs[0.1] < 0.2
+
+
+def test_mixed_steps_stay_close():
+    """10 SSP-RK3 steps: the state error of the precision form stays at the FP32 contraction level."""
+    _require_gpu()
+    mesh, geo, u0 = dg.tgv_3d(8, 3, seed=0, mach=0.1)
+    dt = dg.dt_cfl(mesh, 3, u0)
+    a = dg.Solver(mesh, geo, 3, num_flux="roe")
+    b = dg.Solver(mesh, geo, 3, num_flux="roe", precision="mixed")
+    a.set_state(u0)
+    b.set_state(u0)
+    a.step(dt, 10)
+    b.step(dt, 10)
+    ua, ub = a.get_state(), b.get_state()
+    err = np.linalg.norm(ub - ua) / np.linalg.norm(ua)
+    print(f"state rel L2 after 10 steps {err:.3e}", flush=True)
+    assert 0 < err < 1e-6
+    a.close()
+    b.close()
+
+
+def test_mixed_refused_where_not_built():
+    _require_gpu()
+    mesh, geo, u0 = dg.vortex_2d(4, 2)
+    with pytest.raises(dg._ffi.DgxError):
+        dg.Solver(mesh, geo, 2, num_flux="lf", precision="mixed")
+    s = dg.Solver(mesh, geo, 2, num_flux="lf")
+    s.set_state(u0)
+    with pytest.raises(dg._ffi.DgxError):
+        s.compute_rhs(precision="mixed")
+    s.close()
+    L = 2 * math.pi
+    m3 = dg.structured_mesh(3, (2, 2, 2), (0, 0, 0), (L, L, L), p_g=3)
+    with pytest.raises(dg._ffi.DgxError):
+        dg.Solver(m3, dg.geometry(m3, 3), 3, num_flux="roe", precision="mixed", viscous=(0.01, 0.71))
