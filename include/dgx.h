@@ -35,6 +35,9 @@ typedef enum dgx_status {
 
 enum { DGX_PDE_EULER = 0, DGX_PDE_NAVIER_STOKES = 1 };   /* test_case.c pde_index: euler, navier_stokes */
 enum { DGX_FLUX_LAX_FRIEDRICHS = 0, DGX_FLUX_ROE_PIKE = 1 };   /* numerical_flux_euler_T.c:60-327, 470-778 */
+/* DGX_PREC_FP64: the plain form.  DGX_PREC_MIXED: face traces, weak derivative and face lift contract
+ * FP32 hi/lo parts of the FP64 coefficients with FP32 accumulation (Euler HEX p=3 only; dgx_create and
+ * dgx_rhs return DGX_ERR_INVALID_ARG elsewhere).  No reference counterpart: the reference is FP64. */
 enum { DGX_PREC_FP64 = 0, DGX_PREC_MIXED = 1 };
 enum { DGX_LAYOUT_REF = 0, DGX_LAYOUT_SOA = 1 };               /* SOA = device layout u[v][e][n] */
 enum { DGX_PTR_HOST = 0, DGX_PTR_DEVICE = 1 };
