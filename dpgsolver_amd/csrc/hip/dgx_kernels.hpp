@@ -118,12 +118,30 @@ __device__ inline Dual rcp_nr(Dual x)
     const double r = rcp_nr(x.v);
     return { r, -x.d * r * r };
 }
+// sqrt(x) for x > 0 in the normal range (squared face Jacobians, density ratios, squared sound speeds):
+// the hardware reciprocal square-root estimate, one Goldschmidt step for (g, h) ~ (sqrt x, 1/(2 sqrt x))
+// and two Newton corrections of g (within ~1 ulp).  It skips the scaling for denormal / huge operands
+// and the 0 / inf handling of the library square root, which these operands never need.
+__device__ inline double sqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
 __device__ inline double sqrt_t(double x) { return sqrt(x); }
+__device__ inline double sqrt_pos_t(double x) { return sqrt_pos(x); }
 __device__ inline Dual sqrt_t(Dual x)
 {
     const double s = sqrt(x.v);
     return { s, 0.5 * x.d * rcp_nr(s) };
 }
+__device__ inline Dual sqrt_pos_t(Dual x) { return sqrt_t(x); }
 __device__ inline double neg_if_negative(double x) { return x < 0.0 ? -x : x; }
 __device__ inline Dual neg_if_negative(Dual x) { return x.v < 0.0 ? -x : x; }
 
@@ -165,7 +183,7 @@ __device__ inline void numflux_lf(const T* wl, const T* wr, const double* n, T* 
             V2 = V2 + u * u;
         }
         const T p = kGm1 * (w[D + 1] - 0.5 * rho * V2);
-        lam[s] = sqrt_t(V2) + sqrt_t(kGamma * p * rho_inv);
+        lam[s] = sqrt_t(V2) + sqrt_pos_t(kGamma * p * rho_inv);
     }
     const T maxV = (val(lam[0]) > val(lam[1])) ? lam[0] : lam[1];
     T fl[D + 2], fr[D + 2];
@@ -209,7 +227,7 @@ __device__ inline void numflux_roe(const T* wl, const T* wr, const double* n, T*
     }
     const T pL = kGm1 * (EL - 0.5 * rhoL * V2L), pR = kGm1 * (ER - 0.5 * rhoR * V2R);
     const T HL = (EL + pL) * rhoL_inv, HR = (ER + pR) * rhoR_inv;
-    const T r = sqrt_t(rhoR * rhoL_inv), rP1_inv = rcp_nr(r + 1.0);
+    const T r = sqrt_pos_t(rhoR * rhoL_inv), rP1_inv = rcp_nr(r + 1.0);
     const T rho = r * rhoL;
     T u[3], Vn = zero, V2 = zero;
 #pragma unroll
@@ -220,7 +238,7 @@ __device__ inline void numflux_roe(const T* wl, const T* wr, const double* n, T*
     }
     const T H = (r * HR + HL) * rP1_inv;
     const T c2 = kGm1 * (H - 0.5 * V2);
-    const T c = sqrt_t(c2), c_inv = rcp_nr(c);
+    const T c = sqrt_pos_t(c2), c_inv = rcp_nr(c);
     const T l1 = min_abs_real(VnL - c, Vn - c);
     const T l5 = max_abs_real(VnR + c, Vn + c);
     const T l234 = (val(Vn) > 0.0) ? Vn : -Vn;
