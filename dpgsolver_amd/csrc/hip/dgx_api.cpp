@@ -47,6 +47,8 @@ struct dgx_ctx {
     unsigned long long* bad_elem = nullptr;
     unsigned int* nonfinite = nullptr;
     int cur = 0;                 // index of the trace buffer holding the current state's traces
+    int state_buf = 0;           // HEX p=3: the current state is in u (0) or in up (1) (RK buffer rotation)
+    int rk_next = 0;             // next SSP-RK3 stage dgx_rk_stage accepts
     hipGraphExec_t graph[2] = { nullptr, nullptr };
     double graph_dt = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -121,8 +123,18 @@ hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin,
     if (c->hex3) {
         dgx::Hex3Args a;
         std::memset(&a, 0, sizeof(a));
-        a.u = c->u;
-        a.up = c->up;
+        // buffer rotation: stage 0 u -> up, stage 1 up (+u^n in u) -> up, stage 2 up (+u^n) -> u
+        if (stage < 0) {
+            a.u = c->state_buf ? c->up : c->u;
+        } else if (stage == 0) {
+            a.u = c->u;
+            a.u_out = c->up;
+        } else {
+            a.u = c->up;
+            a.up = c->u;
+            a.u_out = (stage == 1) ? c->up : c->u;
+        }
+
         a.met = c->met;
         a.imass = c->imass;
         a.tr_in = c->tr[tin];
@@ -577,6 +589,8 @@ dgx_status dgx_set_state(dgx_ctx* ctx, int layout, int ptr_kind, const double* u
     if (st != DGX_OK)
         return st;
     ctx->cur = 0;
+    ctx->state_buf = 0;
+    ctx->rk_next = 0;
     if (ctx->hex3)
         HIP_TRY(ctx, dgx::launch_hex3_trace(ctx->u, ctx->tr[0], ctx->slot_elem, ctx->Ep, ctx->stream));
     else
@@ -595,7 +609,7 @@ dgx_status dgx_get_state(dgx_ctx* ctx, int layout, int ptr_kind, double* u)
     if (!u || (layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
         (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_get_state: bad arguments");
-    dgx_status st = download_state(ctx, layout, ptr_kind, ctx->u, u);
+    dgx_status st = download_state(ctx, layout, ptr_kind, ctx->state_buf ? ctx->up : ctx->u, u);
     if (st != DGX_OK)
         return st;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -638,10 +652,16 @@ dgx_status dgx_rk_stage(dgx_ctx* ctx, int stage, double dt, double* max_rhs)
         return DGX_ERR_INVALID_ARG;
     if (stage < 0 || stage > 2 || !std::isfinite(dt))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rk_stage: bad arguments");
+    if (stage != ctx->rk_next)
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rk_stage: stages run in the order 0, 1, 2 (expected stage " +
+                                                  std::to_string(ctx->rk_next) + ")");
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     HIP_TRY(ctx, launch_any(ctx, stage, dt, 1, ctx->cur, ctx->precision));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     ctx->cur ^= 1;
+    ctx->rk_next = (stage + 1) % 3;
+    if (ctx->hex3)
+        ctx->state_buf = (stage == 2) ? 0 : 1;
     dgx_status st = read_max(ctx, max_rhs);
     if (st != DGX_OK)
         return st;
@@ -684,6 +704,8 @@ dgx_status dgx_step(dgx_ctx* ctx, double dt, int nsteps, double* max_rhs)
         return DGX_ERR_INVALID_ARG;
     if (nsteps < 0 || !std::isfinite(dt))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_step: bad arguments");
+    if (ctx->rk_next != 0)
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_step: an SSP-RK3 step started by dgx_rk_stage is unfinished");
     if (nsteps == 0)
         return DGX_OK;
     if (dt != ctx->graph_dt || !ctx->graph[0] || !ctx->graph[1]) {
@@ -741,7 +763,7 @@ static dgx_status lin_apply(dgx_ctx* c, bool adjoint, int apply_minv, int layout
     HIP_TRY(c, dgx::launch_trace(c->d, c->p, c->lin_x, c->lin_tr, c->Ep, c->stream));
     dgx::LinArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.u = c->u;
+    a.u = c->state_buf ? c->up : c->u;
     a.x = c->lin_x;
     a.met = c->met;
     a.imass = c->imass;

@@ -337,8 +337,10 @@ __device__ inline void viscous_flux(const double* w, const double* g, double mu,
 
 // Arguments of the HEX p=3 group kernel; arrays are in slot order (slot = group*8 + wave).
 struct Hex3Args {
-    double* u;                  // [NV][Ep][64]
-    double* up;                 // [NV][Ep][64]
+    const double* u;            // [NV][Ep][64] state the stage starts from (u^n, u^(1) or u^(2))
+    const double* up;           // [NV][Ep][64] u^n, read by stages 1 and 2
+    double* u_out;              // [NV][Ep][64] stage result: stage 0 -> the register buffer, 1 -> in place,
+                                // 2 -> the u^n buffer (in place of u^n, which it no longer needs)
     const double* met;          // [9][Ep][64]
     const double* imass;        // [Ep][64]
     const double* tr_in;        // [Ep][6][NV][16]

@@ -108,7 +108,8 @@ dgx_status dgx_rhs(dgx_ctx* ctx, int precision, int apply_minv, int layout, int 
 
 /* One SSP-RK3 stage (stage 0,1,2) of time_step_ssp_rk_33 (solve_explicit.c:252-301): residual,
  * stage combination, then enforce_positivity_highorder (solve.c:187-233) when enabled.
- * max_rhs (nullable) as for dgx_rhs, of this stage's residual. */
+ * max_rhs (nullable) as for dgx_rhs, of this stage's residual.  Stages run in the order 0, 1, 2, as
+ * time_step_ssp_rk_33 runs them (another order returns DGX_ERR_INVALID_ARG; dgx_set_state restarts). */
 dgx_status dgx_rk_stage(dgx_ctx* ctx, int stage, double dt, double* max_rhs);
 
 /* nsteps full SSP-RK3 steps with fixed dt, launched as a captured hipGraph.  max_rhs (nullable) of
