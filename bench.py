@@ -158,6 +158,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-precision-form", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--traffic-file", default=os.path.join(HERE, "profiles", "stage_hex_p3_64_traffic.md"))
@@ -245,12 +246,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": (f"dgx::ns_gtrace_kernel<{d},{p}> + dgx::ns_stage_kernel<{d},{p},{num_flux}>"
-                                if viscous else f"dgx::hex3_stage_kernel<{num_flux}>" if (d, p) == (3, 3)
+                                if viscous else f"dgx::hex3_stage_kernel<{num_flux},{str(args.precision == 'mixed').lower()}>"
+                                if (d, p) == (3, 3)
                                 else f"dgx::stage_kernel<{d},{p},{num_flux}>"),
                      "algorithmic_bytes_per_launch": bytes_alg, "bytes_per_element": b_elem,
                      "stage_us": t_stage_s * 1e6},
     }
-    if args.config in ("hex64", "hex32") and args.precision == "fp64":
+    if args.config in ("hex64", "hex32") and args.precision == "fp64" and not args.no_precision_form:
         out["precision_form"] = precision_form(args, dg, torch, mesh, geo, u0, dt, solver, ms / stages)
     if viscous:
         # kinetic-energy history of the run (nodal sum of rho |u|^2 / 2 over the whole mesh)
