@@ -245,7 +245,8 @@ def main():
                    "stages_per_step": 3, "precision": args.precision, "dt": dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": (f"dgx::ns_gtrace_kernel<{d},{p}> + dgx::ns_stage_kernel<{d},{p},{num_flux}>"
+                     "kernel": ((f"dgx::ns4_stage_kernel<{num_flux}>" if (d, p) == (3, 4) else
+                                 f"dgx::ns_gtrace_kernel<{d},{p}> + dgx::ns_stage_kernel<{d},{p},{num_flux}>")
                                 if viscous else f"dgx::hex3_stage_kernel<{num_flux},{str(args.precision == 'mixed').lower()}>"
                                 if (d, p) == (3, 3)
                                 else f"dgx::stage_kernel<{d},{p},{num_flux}>"),

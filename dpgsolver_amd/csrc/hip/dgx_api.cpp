@@ -21,8 +21,10 @@ struct dgx_ctx {
     int flux = 0, positivity = 1, precision = 0;
     bool visc = false;           // Navier-Stokes (BR2) instead of Euler
     double mu = 0.0, Pr = 1.0, eta = 0.0;
-    double* gtr = nullptr;       // NS: [Ep][nface][d*nv][nf] face traces of the volume gradient
+    double* gtr[2] = { nullptr, nullptr };  // NS: [Ep][nface][d*nv][nf] face traces of the volume gradient
+                                            // (HEX p=4: two buffers, indexed like tr)
     double* lam = nullptr;       // NS: [Ep][nface][nf] lift trace factors
+    double* fnv = nullptr;       // NS HEX p=4: [Ep][nface][3][nf] |J_f| n, own outward normal
     bool hex3 = false;           // HEX p=3 group kernel
     int64_t E = 0, F = 0;
     int64_t Ep = 0;              // number of slots (E plus padding)
@@ -81,7 +83,7 @@ void free_all(dgx_ctx* c)
     }
     void* ptrs[] = { c->u, c->up, c->rhs, c->stage_buf, c->met, c->imass, c->tr[0], c->tr[1], c->nbr,
                      c->slot_elem, c->elem_slot, c->face_list, c->face_count, c->blockmax, c->dmax,
-                     c->bad_elem, c->nonfinite, c->lin_x, c->lin_tr, c->gtr, c->lam };
+                     c->bad_elem, c->nonfinite, c->lin_x, c->lin_tr, c->gtr[0], c->gtr[1], c->lam, c->fnv };
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -171,12 +173,14 @@ hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin,
     a.apply_minv = apply_minv;
     a.positivity = c->positivity;
     if (c->visc) {
-        a.gtr = c->gtr;
+        a.gtr = c->fnv ? c->gtr[tin] : c->gtr[0];
+        a.gtr_out = c->fnv ? c->gtr[tin ^ 1] : nullptr;
         a.lam = c->lam;
+        a.fnv = c->fnv;
         a.mu = c->mu;
         a.Pr = c->Pr;
         a.eta = c->eta;
-        return dgx::launch_ns_stage(c->d, c->p, c->flux, a, c->gtr, c->stream);
+        return dgx::launch_ns_stage(c->d, c->p, c->flux, a, c->gtr[0], c->stream);
     }
     return dgx::launch_stage(c->d, c->p, c->flux, a, c->stream);
 }
@@ -512,8 +516,12 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         step(hipMalloc(&c->face_count, fcount.size()), "hipMalloc(face_count)");
     }
     if (visc) {
-        step(hipMalloc(&c->gtr, sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "hipMalloc(gtr)");
+        step(hipMalloc(&c->gtr[0], sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "hipMalloc(gtr)");
         step(hipMalloc(&c->lam, sizeof(double) * lam.size()), "hipMalloc(lam)");
+        if (d == 3 && p == 4) {
+            step(hipMalloc(&c->gtr[1], sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "hipMalloc(gtr1)");
+            step(hipMalloc(&c->fnv, sizeof(double) * (size_t)Ep * nface * 3 * nf), "hipMalloc(fnv)");
+        }
     }
     step(hipMalloc(&c->blockmax, sizeof(double) * c->nmax), "hipMalloc(blockmax)");
     step(hipMalloc(&c->dmax, sizeof(double)), "hipMalloc(dmax)");
@@ -539,9 +547,13 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         step(hipMemset(c->blockmax, 0, sizeof(double) * c->nmax), "memset blockmax");
         if (visc) {
             step(hipMemcpy(c->lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice), "copy lam");
-            step(hipMemset(c->gtr, 0, sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "memset gtr");
+            for (int i = 0; i < 2; ++i)
+                if (c->gtr[i])
+                    step(hipMemset(c->gtr[i], 0, sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "memset gtr");
         }
         step(dgx::upload_ops(p, ops), "upload ops");
+        if (c->fnv)
+            step(dgx::launch_ns4_face_normals(c->met, c->fnv, E, c->stream), "face normals");
     }
     if (st == DGX_OK)
         st = reset_flags(c);
@@ -595,6 +607,8 @@ dgx_status dgx_set_state(dgx_ctx* ctx, int layout, int ptr_kind, const double* u
         HIP_TRY(ctx, dgx::launch_hex3_trace(ctx->u, ctx->tr[0], ctx->slot_elem, ctx->Ep, ctx->stream));
     else
         HIP_TRY(ctx, dgx::launch_trace(ctx->d, ctx->p, ctx->u, ctx->tr[0], ctx->E, ctx->stream));
+    if (ctx->fnv)
+        HIP_TRY(ctx, dgx::launch_ns4_gtrace(ctx->u, ctx->met, ctx->imass, ctx->gtr[0], ctx->E, ctx->stream));
     st = reset_flags(ctx);
     if (st != DGX_OK)
         return st;

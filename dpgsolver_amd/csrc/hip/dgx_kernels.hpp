@@ -286,6 +286,8 @@ struct StageArgs {
     // Navier-Stokes only
     const double* gtr;      // [E][NFACE][D*NV][NF] face traces of the volume gradient of the current state
     const double* lam;      // [E][NFACE][NF] lift trace factor w_f(m) sum_k l_f(k)^2 / (w(k) w_f(m) J)(k, m)
+    const double* fnv;      // HEX p=4: [E][NFACE][3][NF] |J_f| n of the own outward normal (set-up)
+    double* gtr_out;        // HEX p=4: gradient traces of the state a stage writes
     double mu, Pr, eta;     // viscosity, Prandtl number, BR2 penalty
 };
 
@@ -333,6 +335,53 @@ __device__ inline void viscous_flux(const double* w, const double* g, double mu,
         }
         f2[i + D * (D + 1)] = -(cE * dTs[i] + ut);
     }
+}
+
+// The same flux projected on a direction nh: fn[eq] = sum_i nh_i f2[i + D*eq], formed without the full
+// (D, NV) array (tau is symmetric, so sum_i nh_i (u.tau)_i = sum_m u_m (nh.tau)_m).
+template <int D>
+__device__ inline void viscous_flux_n(const double* w, const double* g, double mu, double Pr, const double* nh,
+                                      double* fn)
+{
+    constexpr int NV = D + 2;
+    const double rho_inv = rcp_nr(w[0]);
+    double uvw[D], duvw[D][D];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+        uvw[k] = rho_inv * w[1 + k];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            duvw[i][j] = rho_inv * (g[(1 + i) + NV * j] - g[NV * j] * uvw[i]);
+    double divV = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+        divV += duvw[k][k];
+    double ndT = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        const double dE_o_rho = rho_inv * rho_inv * (g[(D + 1) + NV * j] * w[0] - w[D + 1] * g[NV * j]);
+        double dV2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            dV2 += uvw[k] * duvw[k][j];
+        ndT += nh[j] * (dE_o_rho - dV2);
+    }
+    fn[0] = 0.0;
+    double ut = 0.0;
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const double tau = (i == m) ? mu * 2.0 * (duvw[i][i] - divV * (1.0 / 3.0)) : mu * (duvw[m][i] + duvw[i][m]);
+            t += nh[i] * tau;
+        }
+        fn[1 + m] = -t;
+        ut += uvw[m] * t;
+    }
+    fn[D + 1] = -(kGamma / Pr * mu * ndT + ut);
 }
 
 // Arguments of the HEX p=3 group kernel; arrays are in slot order (slot = group*8 + wave).

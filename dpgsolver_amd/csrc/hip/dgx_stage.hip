@@ -650,7 +650,7 @@ int64_t stage_grid(int d, int p, int64_t n_elem)
     int e = 1;
 #define DGX_EPB(DD, PP) (e = epb<DD, PP>(), hipSuccess)
     auto f = [&]() -> hipError_t { DGX_DISPATCH(d, p, DGX_EPB); };
-    f();
+    (void)f();
 #undef DGX_EPB
     return (n_elem + e - 1) / e;
 }
@@ -707,3 +707,4 @@ hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, h
 #include "dgx_hex3.inc"
 #include "dgx_lin.inc"
 #include "dgx_ns.inc"
+#include "dgx_ns4.inc"
