@@ -21,8 +21,9 @@ struct dgx_ctx {
     int flux = 0, positivity = 1, precision = 0;
     bool visc = false;           // Navier-Stokes (BR2) instead of Euler
     double mu = 0.0, Pr = 1.0, eta = 0.0;
-    double* gtr[2] = { nullptr, nullptr };  // NS: [Ep][nface][d*nv][nf] face traces of the volume gradient
-                                            // (HEX p=4: two buffers, indexed like tr)
+    double* gtr[2] = { nullptr, nullptr };  // NS: [Ep][nface][d*nv][nf] face traces of the volume gradient;
+                                            // HEX p=4: [Ep][nface][nv][nf] viscous normal-flux traces,
+                                            // two buffers indexed like tr
     double* lam = nullptr;       // NS: [Ep][nface][nf] lift trace factors
     double* fnv = nullptr;       // NS HEX p=4: [Ep][nface][3][nf] |J_f| n, own outward normal
     bool hex3 = false;           // HEX p=3 group kernel
@@ -492,6 +493,7 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     c->nmax = c->hex3 ? Ep : (visc ? dgx::ns_grid(d, p, E) : dgx::stage_grid(d, p, E));
     const size_t sbytes = sizeof(double) * (size_t)Ep * c->nv * ns;
     const size_t tbytes = sizeof(double) * (size_t)Ep * nface * c->nv * c->nf;
+    const size_t gbytes = (d == 3 && p == 4) ? tbytes : (size_t)d * tbytes;   // gtr
     dgx_status st = DGX_OK;
     auto step = [&](hipError_t e_, const char* what) {
         if (st == DGX_OK && e_ != hipSuccess)
@@ -516,10 +518,10 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         step(hipMalloc(&c->face_count, fcount.size()), "hipMalloc(face_count)");
     }
     if (visc) {
-        step(hipMalloc(&c->gtr[0], sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "hipMalloc(gtr)");
+        step(hipMalloc(&c->gtr[0], gbytes), "hipMalloc(gtr)");
         step(hipMalloc(&c->lam, sizeof(double) * lam.size()), "hipMalloc(lam)");
         if (d == 3 && p == 4) {
-            step(hipMalloc(&c->gtr[1], sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "hipMalloc(gtr1)");
+            step(hipMalloc(&c->gtr[1], gbytes), "hipMalloc(gtr1)");
             step(hipMalloc(&c->fnv, sizeof(double) * (size_t)Ep * nface * 3 * nf), "hipMalloc(fnv)");
         }
     }
@@ -549,7 +551,7 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
             step(hipMemcpy(c->lam, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice), "copy lam");
             for (int i = 0; i < 2; ++i)
                 if (c->gtr[i])
-                    step(hipMemset(c->gtr[i], 0, sizeof(double) * (size_t)Ep * nface * d * c->nv * nf), "memset gtr");
+                    step(hipMemset(c->gtr[i], 0, gbytes), "memset gtr");
         }
         step(dgx::upload_ops(p, ops), "upload ops");
         if (c->fnv)
@@ -608,7 +610,8 @@ dgx_status dgx_set_state(dgx_ctx* ctx, int layout, int ptr_kind, const double* u
     else
         HIP_TRY(ctx, dgx::launch_trace(ctx->d, ctx->p, ctx->u, ctx->tr[0], ctx->E, ctx->stream));
     if (ctx->fnv)
-        HIP_TRY(ctx, dgx::launch_ns4_gtrace(ctx->u, ctx->met, ctx->imass, ctx->gtr[0], ctx->E, ctx->stream));
+        HIP_TRY(ctx, dgx::launch_ns4_vtrace(ctx->u, ctx->met, ctx->imass, ctx->fnv, ctx->mu, ctx->Pr, ctx->gtr[0],
+                                            ctx->E, ctx->stream));
     st = reset_flags(ctx);
     if (st != DGX_OK)
         return st;

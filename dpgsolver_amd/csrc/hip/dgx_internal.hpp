@@ -27,9 +27,10 @@ hipError_t launch_ns_stage(int d, int p, int flux, const StageArgs& args, double
 int64_t ns_grid(int d, int p, int64_t n_elem);
 // HEX p=4 Navier-Stokes: |J_f| n at every face point of every element, from the metric terms (set-up).
 hipError_t launch_ns4_face_normals(const double* met, double* fnv, int64_t E, hipStream_t st);
-// HEX p=4 Navier-Stokes: gradient traces of the state u (after an upload; stages write their own).
-hipError_t launch_ns4_gtrace(const double* u, const double* met, const double* imass, double* gtr, int64_t E,
-                             hipStream_t st);
+// HEX p=4 Navier-Stokes: viscous normal-flux traces n.F_v of the state u (after an upload; stages write
+// their own), [E][6][5][25].
+hipError_t launch_ns4_vtrace(const double* u, const double* met, const double* imass, const double* fnv, double mu,
+                             double Pr, double* vtr, int64_t E, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
 
 }  // namespace dgx

@@ -285,9 +285,10 @@ struct StageArgs {
     int positivity;
     // Navier-Stokes only
     const double* gtr;      // [E][NFACE][D*NV][NF] face traces of the volume gradient of the current state
+                            // (HEX p=4: [E][NFACE][NV][NF] traces of n.f2(w, grad w), outward unit n)
     const double* lam;      // [E][NFACE][NF] lift trace factor w_f(m) sum_k l_f(k)^2 / (w(k) w_f(m) J)(k, m)
     const double* fnv;      // HEX p=4: [E][NFACE][3][NF] |J_f| n of the own outward normal (set-up)
-    double* gtr_out;        // HEX p=4: gradient traces of the state a stage writes
+    double* gtr_out;        // HEX p=4: the n.f2 traces of the state a stage writes
     double mu, Pr, eta;     // viscosity, Prandtl number, BR2 penalty
 };
 
