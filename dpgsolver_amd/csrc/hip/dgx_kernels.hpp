@@ -64,10 +64,12 @@ __device__ inline int face_node(int dn, int a, int m)
     if constexpr (D == 2) {
         return dn == 0 ? a + N1 * m : m + N1 * a;
     } else {
+        // selects, not branches: with dn known only at run time a branch splits every trace loop into
+        // one copy per direction, each issuing all its tile loads before the copies merge
         const int m0 = m % N1, m1 = m / N1;
-        if (dn == 0) return a + N1 * (m0 + N1 * m1);
-        if (dn == 1) return m0 + N1 * (a + N1 * m1);
-        return m0 + N1 * (m1 + N1 * a);
+        const int stride = dn == 0 ? 1 : (dn == 1 ? N1 : N1 * N1);
+        const int base = dn == 0 ? N1 * m : (dn == 1 ? m0 + N1 * N1 * m1 : m);
+        return base + a * stride;
     }
 }
 
