@@ -238,84 +238,96 @@ __device__ inline void stage_tail(const StageArgs& a, int64_t e, int n, int el, 
             for (int v = 0; v < NV; ++v)
                 b[v] = 1.0;
         }
-        bool failed = false, changed = false;
-        double sums[NV], mn;
-        element_reduce<D, P, NV>(s_red, el, n, in_tile, b, b[0], sums, &mn);
-        double avg[NV];
-        if (in_tile) {
+        // Every Bezier coefficient of every element of the block at or above the bound: nothing is
+        // limited and no element fails (an element average is never below its minimum), so the reductions
+        // and the Bezier -> Lagrange transform are skipped and the nodal stage values are kept (the
+        // reference's unconditional round trip changes them only by round-off).  NaN takes the full path.
+        const bool low = valid && !(b[0] >= kEpsPhys && b[NV - 1] >= kEpsPhys);
+        if (__syncthreads_or(low)) {
+            bool failed = false, changed = false;
+            double sums[NV], mn;
+            element_reduce<D, P, NV>(s_red, el, n, in_tile, b, b[0], sums, &mn);
+            double avg[NV];
+            if (in_tile) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-                avg[v] = sums[v] * (1.0 / NS);
-            if (avg[0] < kEpsPhys) {
-                failed = true;
-            } else if (mn < kEpsPhys) {
-                double p_ho = (avg[0] - kEpsPhys) / (avg[0] - mn);
-                p_ho = fmax(0.0, fmin(1.0, p_ho));
+                for (int v = 0; v < NV; ++v)
+                    avg[v] = sums[v] * (1.0 / NS);
+                if (avg[0] < kEpsPhys) {
+                    failed = true;
+                } else if (mn < kEpsPhys) {
+                    double p_ho = (avg[0] - kEpsPhys) / (avg[0] - mn);
+                    p_ho = fmax(0.0, fmin(1.0, p_ho));
 #pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    double av = avg[v];
-                    if ((v == 0 || v == NV - 1) && av < kEpsPhys)
-                        av = kEpsPhys;
-                    b[v] = p_ho * b[v] + (1.0 - p_ho) * av;
-                    avg[v] = p_ho * avg[v] + (1.0 - p_ho) * av;
+                    for (int v = 0; v < NV; ++v) {
+                        double av = avg[v];
+                        if ((v == 0 || v == NV - 1) && av < kEpsPhys)
+                            av = kEpsPhys;
+                        b[v] = p_ho * b[v] + (1.0 - p_ho) * av;
+                        avg[v] = p_ho * avg[v] + (1.0 - p_ho) * av;
+                    }
+                    changed = true;
                 }
-                changed = true;
             }
-        }
-        double sp_sum[1], mnp;
-        element_reduce<D, P, 1>(s_red, el, n, in_tile, &b[NV - 1], b[NV - 1], sp_sum, &mnp);
-        if (in_tile && !failed) {
-            const double avgp = sp_sum[0] * (1.0 / NS);
-            if (avgp < kEpsPhys) {
-                failed = true;
-            } else if (mnp < kEpsPhys) {
-                double p_ho = (avgp - kEpsPhys) / (avgp - mnp);
-                p_ho = fmax(0.0, fmin(1.0, p_ho));
-                avg[NV - 1] = avgp;
+            double sp_sum[1], mnp;
+            element_reduce<D, P, 1>(s_red, el, n, in_tile, &b[NV - 1], b[NV - 1], sp_sum, &mnp);
+            if (in_tile && !failed) {
+                const double avgp = sp_sum[0] * (1.0 / NS);
+                if (avgp < kEpsPhys) {
+                    failed = true;
+                } else if (mnp < kEpsPhys) {
+                    double p_ho = (avgp - kEpsPhys) / (avgp - mnp);
+                    p_ho = fmax(0.0, fmin(1.0, p_ho));
+                    avg[NV - 1] = avgp;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    double av = avg[v];
-                    if ((v == 0 || v == NV - 1) && av < kEpsPhys)
-                        av = kEpsPhys;
-                    b[v] = p_ho * b[v] + (1.0 - p_ho) * av;
+                    for (int v = 0; v < NV; ++v) {
+                        double av = avg[v];
+                        if ((v == 0 || v == NV - 1) && av < kEpsPhys)
+                            av = kEpsPhys;
+                        b[v] = p_ho * b[v] + (1.0 - p_ho) * av;
+                    }
+                    changed = true;
                 }
-                changed = true;
             }
-        }
-        (void)changed;
-        if (valid && failed && n == 0)
-            atomicMin(a.bad_elem, (unsigned long long)e);
-        if (valid && !failed) {
-            // primitive -> conservative (solution_euler_T.c:152-167), then Bezier -> Lagrange (ccBS0)
-            double V2 = 0.0;
+            (void)changed;
+            if (valid && failed && n == 0)
+                atomicMin(a.bad_elem, (unsigned long long)e);
+            if (valid && !failed) {
+                // primitive -> conservative (solution_euler_T.c:152-167), then Bezier -> Lagrange (ccBS0)
+                double V2 = 0.0;
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                V2 += b[1 + k] * b[1 + k];
-                b[1 + k] *= b[0];
+                for (int k = 0; k < D; ++k) {
+                    V2 += b[1 + k] * b[1 + k];
+                    b[1 + k] *= b[0];
+                }
+                b[NV - 1] = b[NV - 1] / kGm1 + 0.5 * b[0] * V2;
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    su[v * NS + n] = b[v];
             }
-            b[NV - 1] = b[NV - 1] / kGm1 + 0.5 * b[0] * V2;
+            __syncthreads();
+            // the element's state is left as is (no round trip) when it failed the check
+            const bool pass_active = valid && !failed;
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-                su[v * NS + n] = b[v];
-        }
-        __syncthreads();
-        // the element's state is left as is (no round trip) when it failed the check
-        const bool pass_active = valid && !failed;
+            for (int dir = 0; dir < D; ++dir)
+                tp_pass_lds<D, P, NV>(su, op.bs, dir, n, pass_active);
+            if (valid && failed) {
+                // restore the un-limited stage result for the traces below
 #pragma unroll
-        for (int dir = 0; dir < D; ++dir)
-            tp_pass_lds<D, P, NV>(su, op.bs, dir, n, pass_active);
-        if (valid && failed) {
-            // restore the un-limited stage result for the traces below
+                for (int v = 0; v < NV; ++v)
+                    su[v * NS + n] = un[v];
+            }
+            __syncthreads();
+            if (valid) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    un[v] = su[v * NS + n];
+            }
+        } else if (valid) {
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 su[v * NS + n] = un[v];
         }
         __syncthreads();
-        if (valid) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-                un[v] = su[v * NS + n];
-        }
     }
 
     if (valid) {
