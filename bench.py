@@ -223,7 +223,9 @@ def main():
     b_elem = algorithmic_bytes_per_element(d, p)
     bytes_alg = b_elem * mesh.n_elem
     achieved = bytes_alg / t_stage_s / 1e9
-    traffic = read_traffic(args.traffic_file) if args.config == "hex64" else None
+    traffic = (read_traffic(args.traffic_file) if args.config == "hex64" else
+               read_traffic(os.path.join(HERE, "profiles", "stage_ns_hex_p4_48_traffic.md"))
+               if args.config == "ns48" else None)
     print(f"[bench] {args.steps} steps in {ms:.2f} ms (wall {wall * 1e3:.2f} ms), stage {t_stage_s * 1e6:.1f} us, "
           f"max_rhs {mx:.4e}", file=sys.stderr, flush=True)
 
