@@ -8,8 +8,9 @@ as one captured hipGraph through the C-ABI (dgx_step); inputs are resident in HB
 Prints one JSON line:
   value    = DOF-stages/s (DOF = E * (p+1)^3 * 5), whole job
   roofline = the stage kernel against HBM: algorithmic bytes per stage launch / measured stage time
-  cpu_baseline = the oracle (dense CPU restatement of the reference algorithm, 1 thread) on a bounded
-             sample of the same workload (a 16^3 mesh of the same construction), in DOF-stages/s
+  cpu_baseline = the oracle (dense CPU restatement of the reference algorithm) on a bounded sample of the
+             same workload (a 16^3 mesh of the same construction), in DOF-stages/s, on all host threads
+             (value, cores) and on one thread (single_thread; the reference is single-threaded)
 """
 import argparse
 import json
@@ -50,8 +51,30 @@ def read_traffic(path):
     return None
 
 
+def host_threads():
+    """Host threads for the all-cores baseline: the CPUs this process may run on, at most
+    OMP_NUM_THREADS and at most 16 (the host share of one GPU on the measurement machines)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(args, d, p, num_flux):
-    """Time the oracle's rk stages on a 16^3 (3-D) / 32^2 (2-D) sample, single thread."""
+    """Time the oracle's rk stages on a 16^3 (3-D) / 32^2 (2-D) sample: one thread (the reference is
+    single-threaded) and all host threads (OpenMP over volumes, faces in two phases)."""
     import numpy as np
     import dpgsolver_amd as dg
     from oracle.oracle import Oracle
@@ -65,18 +88,28 @@ def cpu_baseline(args, d, p, num_flux):
         sample = f"32^2 QUAD p={p}, isentropic vortex, LF, SSP-RK3 + positivity"
     orc = Oracle(mesh, p, num_flux=num_flux, geometry=geo)
     dt = dg.dt_cfl(mesh, p, u0)
-    u, up = u0.copy(), np.zeros_like(u0)
     dof = mesh.n_elem * (p + 1) ** d * (d + 2)
-    stages = 0
-    t0 = time.perf_counter()
-    while True:
-        orc.rk_stage(stages % 3, dt, u, up)
-        stages += 1
-        el = time.perf_counter() - t0
-        if el > args.cpu_seconds and stages % 3 == 0:
-            break
-    return {"value": dof * stages / el, "unit": "DOF-stages/s", "cores": 1, "kind": "port",
-            "sample": f"{sample}; {stages} stages in {el:.1f} s (oracle, dense per-element operators)"}
+
+    def rate(threads, seconds):
+        orc.set_threads(threads)
+        u, up = u0.copy(), np.zeros_like(u0)
+        stages = 0
+        t0 = time.perf_counter()
+        while True:
+            orc.rk_stage(stages % 3, dt, u, up)
+            stages += 1
+            el = time.perf_counter() - t0
+            if el > seconds and stages % 3 == 0:
+                return dof * stages / el, stages, el
+
+    r1, s1, e1 = rate(1, args.cpu_seconds)
+    nt = host_threads()
+    rn, sn, en = rate(nt, args.cpu_seconds / 2) if nt > 1 else (r1, s1, e1)
+    host = f"{cpu_model()}, {os.cpu_count()} logical CPUs, {len(os.sched_getaffinity(0))} available"
+    return {"value": rn, "unit": "DOF-stages/s", "cores": nt, "kind": "port",
+            "sample": f"{sample}; {sn} stages in {en:.1f} s on {nt} threads (oracle, dense per-element "
+                      f"operators; {host})",
+            "single_thread": {"value": r1, "cores": 1, "sample": f"{s1} stages in {e1:.1f} s"}}
 
 
 def precision_form(args, dg, torch, mesh, geo, u0, dt, fp64_solver, fp64_stage_ms):

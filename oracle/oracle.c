@@ -42,6 +42,10 @@ struct orc_ctx {
     int ns;
     double mu, Pr;
     double eta;                 /* BR2 penalty override (> 0), else 1.01 x NFMAX */
+    /* threads of the residual and stage loops (1: the reference's sequential order) */
+    int nthreads;
+    int64_t* ef_face;           /* [E][2d] face of each local face, built on first threaded use */
+    int8_t* ef_side;            /* [E][2d] 0: the volume is the face's left volume, 1: its right */
 };
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -297,6 +301,23 @@ static int ipow(int b, int e)
     for (int i = 0; i < e; ++i)
         r *= b;
     return r;
+}
+
+/* Local face -> (face, side) of every volume, for the threaded residual. */
+static void build_elem_faces(orc_ctx* c)
+{
+    if (c->ef_face)
+        return;
+    const int nfe = 2 * c->d;
+    c->ef_face = (int64_t*)malloc(sizeof(int64_t) * (size_t)c->n_elem * nfe);
+    c->ef_side = (int8_t*)malloc((size_t)c->n_elem * nfe);
+    for (int64_t fi = 0; fi < c->n_face; ++fi)
+        for (int side = 0; side < 2; ++side) {
+            const int64_t e = c->face_elem[2 * fi + side];
+            const int lf = c->face_lf[2 * fi + side];
+            c->ef_face[e * nfe + lf] = fi;
+            c->ef_side[e * nfe + lf] = (int8_t)side;
+        }
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -643,6 +664,7 @@ void orc_destroy(orc_ctx* c)
     free(c->tw0_vt_vc);
     free(c->ccSB0); free(c->ccBS0); free(c->w_vc);
     free(c->metrics_vc); free(c->jdet_vc); free(c->normals_fc); free(c->jdet_fc); free(c->m_inv);
+    free(c->ef_face); free(c->ef_side);
     free(c);
 }
 
@@ -849,6 +871,33 @@ int64_t orc_positivity(orc_ctx* c, double* u)
     return bad;
 }
 
+/* SSP-RK3 combination (time_step_ssp_rk_33, solve_explicit.c:271-293) and positivity of volume e; 0 when
+ * the volume fails the positivity check. */
+static int rk_elem(const orc_ctx* c, int stage, double dt, int64_t e, double* u, double* u_p, const double* rhs,
+                   double* sb)
+{
+    const int nv = c->d + 2, Ns = c->Ns;
+    double* s = u + e * nv * Ns;
+    double* sp = u_p + e * nv * Ns;
+    const double* r = rhs + e * nv * Ns;
+    for (int i = 0; i < nv * Ns; ++i) {
+        if (stage == 0) {
+            sp[i] = s[i];
+            s[i] += dt * r[i];
+        } else if (stage == 1) {
+            s[i] = (1.0 / 4.0) * (3.0 * sp[i] + s[i] + dt * r[i]);
+        } else {
+            s[i] = (1.0 / 3.0) * (sp[i] + 2.0 * s[i] + 2.0 * dt * r[i]);
+        }
+    }
+    return !c->positivity || positivity_elem(c, s, sb);
+}
+
+void orc_set_threads(orc_ctx* c, int nthreads)
+{
+    c->nthreads = nthreads > 1 ? nthreads : 1;
+}
+
 double orc_rk_stage(orc_ctx* c, int stage, double dt, double* u, double* u_p, int64_t* bad_elem)
 {
     const int nv = c->d + 2, Ns = c->Ns;
@@ -857,23 +906,29 @@ double orc_rk_stage(orc_ctx* c, int stage, double dt, double* u, double* u_p, in
     double* sb = (double*)malloc(sizeof(double) * Ns * nv);
     const double max_rhs = residual_d(c, u, rhs, 1);
     int64_t bad = -1;
-    for (int64_t e = 0; e < c->n_elem; ++e) {
-        double* s = u + e * nv * Ns;
-        double* sp = u_p + e * nv * Ns;
-        const double* r = rhs + e * nv * Ns;
-        /* time_step_ssp_rk_33 (solve_explicit.c:271-293) */
-        for (int i = 0; i < nv * Ns; ++i) {
-            if (stage == 0) {
-                sp[i] = s[i];
-                s[i] += dt * r[i];
-            } else if (stage == 1) {
-                s[i] = (1.0 / 4.0) * (3.0 * sp[i] + s[i] + dt * r[i]);
-            } else {
-                s[i] = (1.0 / 3.0) * (sp[i] + 2.0 * s[i] + 2.0 * dt * r[i]);
+    if (c->nthreads <= 1) {
+        for (int64_t e = 0; e < c->n_elem; ++e)
+            if (!rk_elem(c, stage, dt, e, u, u_p, rhs, sb) && bad < 0)
+                bad = e;
+    } else {
+        /* threaded (timing baseline only): the first failing volume is the smallest failing index */
+        int64_t first = c->n_elem;
+#pragma omp parallel num_threads(c->nthreads)
+        {
+            double* sbt = (double*)malloc(sizeof(double) * Ns * nv);
+            int64_t mine = c->n_elem;
+#pragma omp for schedule(static)
+            for (int64_t e = 0; e < c->n_elem; ++e)
+                if (!rk_elem(c, stage, dt, e, u, u_p, rhs, sbt) && e < mine)
+                    mine = e;
+#pragma omp critical
+            {
+                if (mine < first)
+                    first = mine;
             }
+            free(sbt);
         }
-        if (c->positivity && !positivity_elem(c, s, sb) && bad < 0)
-            bad = e;
+        bad = first < c->n_elem ? first : -1;
     }
     if (bad_elem)
         *bad_elem = bad;

@@ -60,6 +60,10 @@ void orc_set_viscous(orc_ctx* c, double mu, double Pr);
 
 /* Test hook: BR2 penalty used instead of 1.01 x NFMAX (NFMAX = 4 for QUAD, 6 for HEX). */
 void orc_set_penalty(orc_ctx* c, double eta);
+/* Threads of orc_residual / orc_rk_stage (OpenMP; default 1).  With more than one thread the face terms
+ * are accumulated per volume in local-face order instead of the reference's face order (round-off
+ * differences only); used to time the baseline on all host cores. */
+void orc_set_threads(orc_ctx* c, int nthreads);
 
 /* Weak gradients of the Navier-Stokes path at state u, each [E][d][nv][Ns]: grad_v (volume term) and
  * grad (volume term plus the liftings of all faces of the element). */

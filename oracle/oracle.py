@@ -30,6 +30,7 @@ _SIGS = {
     "orc_set_viscous": (None, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "orc_gradient": (None, [ctypes.c_void_p, _dp, _dp, _dp]),
     "orc_set_penalty": (None, [ctypes.c_void_p, ctypes.c_double]),
+    "orc_set_threads": (None, [ctypes.c_void_p, ctypes.c_int]),
     "orc_operator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _dp]),
     "orc_residual": (ctypes.c_double, [ctypes.c_void_p, _dp, _dp, ctypes.c_int]),
     "orc_jvp_cs": (None, [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp, ctypes.c_int]),
@@ -177,6 +178,11 @@ class Oracle:
     def set_penalty(self, eta):
         """Test hook: BR2 penalty eta instead of 1.01 x faces per element."""
         load().orc_set_penalty(self._ptr, float(eta))
+
+    def set_threads(self, n):
+        """OpenMP threads of residual / rk_stage (1 = the reference's sequential order; more threads
+        change the face-term summation order, i.e. round-off only)."""
+        load().orc_set_threads(self._ptr, int(n))
 
     def gradient(self, u):
         """(grad_v, grad), each [E][d][nv][Ns]: weak volume gradient and the lifted gradient."""

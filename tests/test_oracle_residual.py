@@ -161,3 +161,26 @@ def test_bezier_maps_closed_form():
     sb = orc.operator("ccSB0").reshape(16, 16)
     assert np.allclose(bs.sum(axis=1), 1.0, atol=1e-14)
     assert np.allclose(bs @ sb, np.eye(16), atol=1e-12)
+
+
+@pytest.mark.parametrize("viscous", [None, (0.05, 0.71)])
+def test_threaded_oracle_matches_sequential(viscous):
+    """The threaded oracle (timing baseline on all host cores) computes the same residual and SSP-RK3
+    stages as the sequential one; only the order of the face-term sums differs (round-off)."""
+    mesh = _curved(3, 3, 2)
+    geo = dg.geometry(mesh, 2)
+    u = dg.ic_taylor_green(mesh, 2, 0.3) + 1e-3 * dg.randn(7, (mesh.n_elem, 5, 27))
+    seq = Oracle(mesh, 2, num_flux=1, geometry=geo, viscous=viscous)
+    par = Oracle(mesh, 2, num_flux=1, geometry=geo, viscous=viscous)
+    par.set_threads(4)
+    for minv in (False, True):
+        r1, m1 = seq.residual(u, apply_minv=minv)
+        r2, m2 = par.residual(u, apply_minv=minv)
+        assert np.linalg.norm(r2 - r1) <= 1e-13 * np.linalg.norm(r1)
+        assert abs(m2 - m1) <= 1e-13 * m1
+    dt = 0.5 * dg.dt_cfl(mesh, 2, u)
+    u1, up1 = u.copy(), np.zeros_like(u)
+    u2, up2 = u.copy(), np.zeros_like(u)
+    seq.step(dt, u1, up1, nsteps=2)
+    par.step(dt, u2, up2, nsteps=2)
+    assert np.linalg.norm(u2 - u1) <= 1e-13 * np.linalg.norm(u1)
