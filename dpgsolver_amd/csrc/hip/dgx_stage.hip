@@ -421,18 +421,18 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs a)
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 wn[v] = src[v * NF];
-            double nh[D];
-            if (!right) {
+            // the left volume's state first and its outward normal; arguments are selected, not branched
+            // on, since the lanes of a wavefront serve left and right sides of faces
+            double nh[D], wl[NV], wr[NV];
 #pragma unroll
-                for (int i = 0; i < D; ++i)
-                    nh[i] = nv[i] * jf_inv;
-                numflux<D, FLUX>(wo, wn, nh, nf);
-            } else {
+            for (int i = 0; i < D; ++i)
+                nh[i] = right ? -nv[i] * jf_inv : nv[i] * jf_inv;
 #pragma unroll
-                for (int i = 0; i < D; ++i)
-                    nh[i] = -nv[i] * jf_inv;
-                numflux<D, FLUX>(wn, wo, nh, nf);
+            for (int v = 0; v < NV; ++v) {
+                wl[v] = right ? wn[v] : wo[v];
+                wr[v] = right ? wo[v] : wn[v];
             }
+            numflux<D, FLUX>(wl, wr, nh, nf);
             const double scale = (right ? -jf : jf) * face_weight<D, P>(op, m);
 #pragma unroll
             for (int v = 0; v < NV; ++v)
