@@ -6,9 +6,9 @@ the host-side set-up helpers and the solver mirror.
 """
 from . import _ffi
 from .problem import (Geometry, Mesh, dt_cfl, geometry, ic_isentropic_vortex, ic_taylor_green, randn,
-                      solution_nodes_xyz, structured_mesh, tgv_3d, vortex_2d)
+                      read_gmsh, solution_nodes_xyz, structured_mesh, tgv_3d, vortex_2d)
 from .solver import Solver, residual_function
 
 __all__ = ["Mesh", "Geometry", "Solver", "structured_mesh", "geometry", "ic_isentropic_vortex",
            "ic_taylor_green", "dt_cfl", "randn", "solution_nodes_xyz", "vortex_2d", "tgv_3d",
-           "residual_function"]
+           "residual_function", "read_gmsh"]

@@ -95,6 +95,8 @@ SIGNATURES = {
     "dgx_mesh_structured": (ctypes.c_int, [ctypes.c_int, c_int32_p, c_double_p, c_double_p, ctypes.c_int,
                                            ctypes.c_double, ctypes.c_uint64, c_double_p, c_int32_p, c_int8_p,
                                            c_int8_p]),
+    "dgx_mesh_read_gmsh": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_int64_p, c_int64_p, c_double_p,
+                                          c_int32_p, c_int8_p, c_int8_p, c_int32_p, c_int32_p, c_int32_p]),
     "dgx_geometry_build": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, c_double_p, c_double_p,
                                           c_double_p, c_double_p]),
     "dgx_solution_nodes_xyz": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, c_double_p]),

@@ -8,6 +8,7 @@
  * in Operator::ops_tp (src/simulation/operators/operator.h:40-44) but leaves disabled on its hot path.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -374,16 +375,21 @@ dgx_status dgx_mesh_structured(int d, const int* n, const double* lo, const doub
                 xe[k * Ng + a] = xk;
             }
         }
-        /* faces on the +dir side of this element */
+        /* the face on the +dir side of this element.  Its left volume is the one the reference's face
+         * construction meets first in volume order (constructor_Faces, face.c:109-146): this element for
+         * an interior face, the neighbour across the periodic boundary (through its -dir face) for the
+         * wrap-around face.  The Roe-Pike entropy fix is not symmetric in left and right, so the choice
+         * is part of the result. */
         for (int dir = 0; dir < d; ++dir) {
             int jc[3] = { ix, iy, iz };
+            const int wrap = (jc[dir] + 1 == n[dir]);
             jc[dir] = (jc[dir] + 1) % n[dir];
             const int64_t er = jc[0] + (int64_t)n[0] * (jc[1] + (int64_t)n1 * jc[2]);
             const int64_t f = dir * E + e;
-            face_elem[2 * f + 0] = (int32_t)e;
-            face_elem[2 * f + 1] = (int32_t)er;
-            face_lf[2 * f + 0] = (int8_t)(2 * dir + 1);
-            face_lf[2 * f + 1] = (int8_t)(2 * dir);
+            face_elem[2 * f + 0] = (int32_t)(wrap ? er : e);
+            face_elem[2 * f + 1] = (int32_t)(wrap ? e : er);
+            face_lf[2 * f + 0] = (int8_t)(wrap ? 2 * dir : 2 * dir + 1);
+            face_lf[2 * f + 1] = (int8_t)(wrap ? 2 * dir + 1 : 2 * dir);
             face_ord[f] = 0;
         }
     }
@@ -697,4 +703,409 @@ double dgx_dt_cfl(int d, int p, int64_t n_elem, const double* u, double h_min, d
     /* the d in the denominator sums the directional CFL numbers; without it SSP-RK3 at p=3 in 3-D is
      * unstable at cfl = 0.5 (the oracle blows up after ~15 steps on a curved 8^3 mesh) */
     return cfl * h_min / ((2.0 * p + 1.0) * d * smax);
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* Gmsh 2.2 ASCII mesh ingest                                                                        */
+/* ------------------------------------------------------------------------------------------------ */
+
+/* Nodes of a Gmsh element type (the types a QUAD/HEX mesh file holds; 0 = unsupported). */
+static int gmsh_type_nodes(int type)
+{
+    switch (type) {
+    case 15: return 1;   /* point */
+    case 1: return 2;    /* line */
+    case 2: return 3;    /* triangle */
+    case 3: return 4;    /* quadrangle */
+    case 4: return 4;    /* tetrahedron */
+    case 5: return 8;    /* hexahedron */
+    case 6: return 6;    /* prism */
+    case 7: return 5;    /* pyramid */
+    default: return 0;
+    }
+}
+
+typedef struct {
+    int type, ntag, tag0, tag1;
+    int nodes[8];   /* node indices (0-based, file order) */
+} gm_elem;
+
+typedef struct {
+    int slave, master;    /* entity tags */
+    int n;
+    int* s;               /* slave node indices */
+    int* m;               /* master node indices */
+} gm_per;
+
+typedef struct {
+    int key[4];           /* sorted (substituted) vertices of a volume's local face */
+    int64_t v;
+    int lf;
+} gm_fkey;
+
+static int cmp_key(const int* x, const int* y);
+
+static int cmp_fkey(const void* a, const void* b)
+{
+    const gm_fkey* x = (const gm_fkey*)a;
+    const gm_fkey* y = (const gm_fkey*)b;
+    const int c = cmp_key(x->key, y->key);
+    if (c != 0)
+        return c;
+    if (x->v != y->v)
+        return x->v < y->v ? -1 : 1;
+    return x->lf - y->lf;
+}
+
+static int cmp_key(const int* x, const int* y)
+{
+    for (int k = 0; k < 4; ++k)
+        if (x[k] != y[k])
+            return x[k] < y[k] ? -1 : 1;
+    return 0;
+}
+
+static void sort_small(int* a, int n)
+{
+    for (int i = 1; i < n; ++i)
+        for (int j = i; j > 0 && a[j - 1] > a[j]; --j) {
+            const int t = a[j];
+            a[j] = a[j - 1];
+            a[j - 1] = t;
+        }
+}
+
+/* Vertices of local face lf = 2*dn + side of a tensor-product element (vertex i0 + 2 i1 + 4 i2), in
+ * tensor-product order over the remaining directions. */
+static int tp_face_vertices(int d, int lf, int* out)
+{
+    const int dn = lf >> 1, side = lf & 1;
+    int k = 0;
+    for (int v = 0; v < (1 << d); ++v)
+        if (((v >> dn) & 1) == side)
+            out[k++] = v;
+    return k;
+}
+
+typedef struct {
+    int d;
+    int64_t n_nodes, n_vol;
+    double* xyz;          /* [n_nodes][3] */
+    int* node_of_id;      /* Gmsh node id -> index */
+    int max_id;
+    gm_elem* el;
+    int64_t n_el;
+    gm_per* per;
+    int n_per;
+} gm_file;
+
+static void gm_free(gm_file* g)
+{
+    free(g->xyz);
+    free(g->node_of_id);
+    free(g->el);
+    for (int i = 0; i < g->n_per; ++i) {
+        free(g->per[i].s);
+        free(g->per[i].m);
+    }
+    free(g->per);
+    memset(g, 0, sizeof(*g));
+}
+
+static int gm_node(const gm_file* g, long id)
+{
+    return (id >= 0 && id <= g->max_id) ? g->node_of_id[id] : -1;
+}
+
+/* Reads $MeshFormat (2.2 ASCII), $Nodes, $Elements and $Periodic (mesh_readers.c:215-440; periodic
+ * entities mesh_readers.c, mesh_periodic.c:102-140). */
+static dgx_status gm_read(const char* path, int d, gm_file* g)
+{
+    memset(g, 0, sizeof(*g));
+    g->d = d;
+    FILE* fp = fopen(path, "r");
+    if (!fp)
+        return DGX_ERR_INVALID_ARG;
+    char tok[256];
+    dgx_status st = DGX_OK;
+    int have_nodes = 0, have_elems = 0;
+    while (st == DGX_OK && fscanf(fp, "%255s", tok) == 1) {
+        if (strcmp(tok, "$MeshFormat") == 0) {
+            double ver;
+            int ftype, dsize;
+            if (fscanf(fp, "%lf %d %d", &ver, &ftype, &dsize) != 3 || ver < 2.0 || ver >= 3.0 || ftype != 0)
+                st = DGX_ERR_INVALID_ARG;   /* only the ASCII 2.x format the reference reads */
+        } else if (strcmp(tok, "$Nodes") == 0) {
+            long n;
+            if (fscanf(fp, "%ld", &n) != 1 || n <= 0) {
+                st = DGX_ERR_INVALID_ARG;
+                break;
+            }
+            g->n_nodes = n;
+            g->xyz = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+            long* ids = (long*)malloc(sizeof(long) * (size_t)n);
+            long max_id = 0;
+            for (long i = 0; i < n && st == DGX_OK; ++i) {
+                if (fscanf(fp, "%ld %lf %lf %lf", &ids[i], &g->xyz[3 * i], &g->xyz[3 * i + 1], &g->xyz[3 * i + 2]) != 4 ||
+                    ids[i] < 0 || ids[i] > 100000000L)
+                    st = DGX_ERR_INVALID_ARG;
+                else if (ids[i] > max_id)
+                    max_id = ids[i];
+            }
+            if (st == DGX_OK) {
+                g->max_id = (int)max_id;
+                g->node_of_id = (int*)malloc(sizeof(int) * (size_t)(max_id + 1));
+                for (long i = 0; i <= max_id; ++i)
+                    g->node_of_id[i] = -1;
+                for (long i = 0; i < n; ++i)
+                    g->node_of_id[ids[i]] = (int)i;
+                have_nodes = 1;
+            }
+            free(ids);
+        } else if (strcmp(tok, "$Elements") == 0) {
+            long n;
+            if (!have_nodes || fscanf(fp, "%ld", &n) != 1 || n <= 0) {
+                st = DGX_ERR_INVALID_ARG;
+                break;
+            }
+            g->n_el = n;
+            g->el = (gm_elem*)calloc((size_t)n, sizeof(gm_elem));
+            for (long i = 0; i < n && st == DGX_OK; ++i) {
+                long id;
+                int type, ntag;
+                gm_elem* e = &g->el[i];
+                if (fscanf(fp, "%ld %d %d", &id, &type, &ntag) != 3 || ntag < 0 || gmsh_type_nodes(type) == 0) {
+                    st = DGX_ERR_INVALID_ARG;
+                    break;
+                }
+                e->type = type;
+                e->ntag = ntag;
+                for (int t = 0; t < ntag; ++t) {
+                    int tv;
+                    if (fscanf(fp, "%d", &tv) != 1) {
+                        st = DGX_ERR_INVALID_ARG;
+                        break;
+                    }
+                    if (t == 0) e->tag0 = tv;
+                    if (t == 1) e->tag1 = tv;
+                }
+                for (int k = 0; k < gmsh_type_nodes(type) && st == DGX_OK; ++k) {
+                    long nid;
+                    if (fscanf(fp, "%ld", &nid) != 1 || (e->nodes[k] = gm_node(g, nid)) < 0)
+                        st = DGX_ERR_INVALID_ARG;
+                }
+            }
+            have_elems = (st == DGX_OK);
+        } else if (strcmp(tok, "$Periodic") == 0) {
+            int n;
+            if (!have_nodes || fscanf(fp, "%d", &n) != 1 || n < 0) {
+                st = DGX_ERR_INVALID_ARG;
+                break;
+            }
+            g->per = (gm_per*)calloc((size_t)(n > 0 ? n : 1), sizeof(gm_per));
+            for (int i = 0; i < n && st == DGX_OK; ++i) {
+                int dim, np;
+                gm_per* p = &g->per[g->n_per++];
+                if (fscanf(fp, "%d %d %d %d", &dim, &p->slave, &p->master, &np) != 4 || np < 0) {
+                    st = DGX_ERR_INVALID_ARG;
+                    break;
+                }
+                p->s = (int*)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+                p->m = (int*)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+                p->n = np;
+                for (int k = 0; k < np && st == DGX_OK; ++k) {
+                    long a, b;
+                    if (fscanf(fp, "%ld %ld", &a, &b) != 2 || (p->s[k] = gm_node(g, a)) < 0 ||
+                        (p->m[k] = gm_node(g, b)) < 0)
+                        st = DGX_ERR_INVALID_ARG;
+                }
+                if (dim != d - 1)
+                    p->n = 0;   /* only face entities carry face node correspondences */
+            }
+        }
+    }
+    fclose(fp);
+    if (st == DGX_OK && !have_elems)
+        st = DGX_ERR_INVALID_ARG;
+    if (st != DGX_OK)
+        gm_free(g);
+    return st;
+}
+
+dgx_status dgx_mesh_read_gmsh(const char* path, int d, int64_t* n_elem, int64_t* n_face, double* xyz_g,
+                              int32_t* face_elem, int8_t* face_lf, int8_t* face_ord, int32_t* v_to_v,
+                              int32_t* v_to_lf, int32_t* node_nums)
+{
+    if (!path || (d != 2 && d != 3) || !n_elem || !n_face)
+        return DGX_ERR_INVALID_ARG;
+    gm_file g;
+    dgx_status st = gm_read(path, d, &g);
+    if (st != DGX_OK)
+        return st;
+    const int vol_type = (d == 2) ? 3 : 5, face_type = (d == 2) ? 1 : 3;
+    const int nvert = 1 << d, nfv = 1 << (d - 1), nlf = 2 * d;
+    /* Gmsh -> tensor-product vertex order (mesh_readers.c:132-162) */
+    static const int ord2[4] = { 0, 1, 3, 2 };
+    static const int ord3[8] = { 0, 1, 3, 2, 4, 5, 7, 6 };
+    const int* ord = (d == 2) ? ord2 : ord3;
+
+    int64_t E = 0;
+    for (int64_t i = 0; i < g.n_el; ++i) {
+        const int t = g.el[i].type;
+        const int dim = (t == 15) ? 0 : (t == 1) ? 1 : (t == 2 || t == 3) ? 2 : 3;
+        if (dim == d && t != vol_type)
+            st = DGX_ERR_INVALID_ARG;   /* mixed element types: not built */
+        if (t == vol_type)
+            ++E;
+    }
+    if (st != DGX_OK || E == 0) {
+        gm_free(&g);
+        return DGX_ERR_INVALID_ARG;
+    }
+    int* vn = (int*)malloc(sizeof(int) * (size_t)E * nvert);   /* volume vertices, TP order */
+    {
+        int64_t e = 0;
+        for (int64_t i = 0; i < g.n_el; ++i)
+            if (g.el[i].type == vol_type) {
+                for (int k = 0; k < nvert; ++k)
+                    vn[e * nvert + k] = g.el[i].nodes[ord[k]];
+                ++e;
+            }
+    }
+    /* boundary entities of the face elements: sorted vertices -> (physical tag, entity tag) */
+    int64_t nb = 0;
+    for (int64_t i = 0; i < g.n_el; ++i)
+        nb += (g.el[i].type == face_type);
+    gm_fkey* bkeys = (gm_fkey*)calloc((size_t)(nb > 0 ? nb : 1), sizeof(gm_fkey));
+    {
+        int64_t b = 0;
+        for (int64_t i = 0; i < g.n_el; ++i)
+            if (g.el[i].type == face_type) {
+                for (int k = 0; k < 4; ++k)
+                    bkeys[b].key[k] = (k < nfv) ? g.el[i].nodes[k] : -1;
+                sort_small(bkeys[b].key, nfv);
+                bkeys[b].v = i;   /* element index in the file */
+                ++b;
+            }
+        qsort(bkeys, (size_t)nb, sizeof(gm_fkey), cmp_fkey);
+    }
+    /* local faces with slave-entity vertices replaced by their masters (correct_f_ve_for_periodic,
+     * mesh_periodic.c:102-140) */
+    gm_fkey* fk = (gm_fkey*)malloc(sizeof(gm_fkey) * (size_t)E * nlf);
+    int* fv_sub = (int*)malloc(sizeof(int) * (size_t)E * nlf * nfv);   /* substituted, TP face order */
+    int* bc = (int*)malloc(sizeof(int) * (size_t)E * nlf);              /* physical tag of a boundary face */
+    for (int64_t e = 0; e < E; ++e)
+        for (int lf = 0; lf < nlf; ++lf) {
+            int loc[4], ids[4] = { -1, -1, -1, -1 };
+            tp_face_vertices(d, lf, loc);
+            for (int k = 0; k < nfv; ++k)
+                ids[k] = vn[e * nvert + loc[k]];
+            gm_fkey probe;
+            memcpy(probe.key, ids, sizeof(ids));
+            sort_small(probe.key, nfv);
+            /* the boundary element on this face, if any */
+            int64_t lo = 0, hi = nb;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if (cmp_key(bkeys[mid].key, probe.key) < 0)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            int phys = 0, ent = 0;
+            if (lo < nb) {
+                int same = 1;
+                for (int k = 0; k < nfv; ++k)
+                    same &= (bkeys[lo].key[k] == probe.key[k]);
+                if (same) {
+                    phys = g.el[bkeys[lo].v].tag0;
+                    ent = g.el[bkeys[lo].v].tag1;
+                }
+            }
+            bc[e * nlf + lf] = phys;
+            for (int p = 0; p < g.n_per && ent != 0; ++p) {
+                if (g.per[p].slave != ent || g.per[p].n == 0)
+                    continue;
+                for (int k = 0; k < nfv; ++k)
+                    for (int q = 0; q < g.per[p].n; ++q)
+                        if (g.per[p].s[q] == ids[k]) {
+                            ids[k] = g.per[p].m[q];
+                            break;
+                        }
+                break;
+            }
+            gm_fkey* f = &fk[e * nlf + lf];
+            memcpy(f->key, ids, sizeof(ids));
+            memcpy(&fv_sub[(e * nlf + lf) * nfv], ids, sizeof(int) * nfv);
+            sort_small(f->key, nfv);
+            f->v = e;
+            f->lf = lf;
+        }
+    qsort(fk, (size_t)E * nlf, sizeof(gm_fkey), cmp_fkey);
+    int32_t* vv = (int32_t*)malloc(sizeof(int32_t) * (size_t)E * nlf);
+    int32_t* vl = (int32_t*)malloc(sizeof(int32_t) * (size_t)E * nlf);
+    for (int64_t i = 0; i < E * nlf; ++i)
+        vv[i] = -1;
+    for (int64_t i = 0; i < E * nlf && st == DGX_OK;) {
+        int64_t j = i + 1;
+        while (j < E * nlf && cmp_key(fk[j].key, fk[i].key) == 0)
+            ++j;
+        if (j - i == 2) {
+            const gm_fkey *a = &fk[i], *b = &fk[i + 1];
+            vv[a->v * nlf + a->lf] = (int32_t)b->v;
+            vl[a->v * nlf + a->lf] = b->lf;
+            vv[b->v * nlf + b->lf] = (int32_t)a->v;
+            vl[b->v * nlf + b->lf] = a->lf;
+        } else if (j - i == 1) {
+            vl[fk[i].v * nlf + fk[i].lf] = bc[fk[i].v * nlf + fk[i].lf];   /* boundary: its condition */
+        } else {
+            st = DGX_ERR_INVALID_ARG;   /* a face shared by more than two volumes */
+        }
+        i = j;
+    }
+    /* faces in the order the volumes meet them: the first volume is the face's left volume */
+    int64_t F = 0;
+    for (int64_t e = 0; e < E && st == DGX_OK; ++e)
+        for (int lf = 0; lf < nlf; ++lf) {
+            const int64_t nb_e = vv[e * nlf + lf];
+            const int nb_lf = vl[e * nlf + lf];
+            if (nb_e < 0 || nb_e < e || (nb_e == e && nb_lf < lf))
+                continue;
+            if (face_elem) {
+                face_elem[2 * F] = (int32_t)e;
+                face_elem[2 * F + 1] = (int32_t)nb_e;
+                face_lf[2 * F] = (int8_t)lf;
+                face_lf[2 * F + 1] = (int8_t)nb_lf;
+                /* aligned when both volumes list the face's (substituted) vertices in the same order */
+                const int* a = &fv_sub[(e * nlf + lf) * nfv];
+                const int* b = &fv_sub[(nb_e * nlf + nb_lf) * nfv];
+                int same = 1;
+                for (int k = 0; k < nfv; ++k)
+                    same &= (a[k] == b[k]);
+                face_ord[F] = same ? 0 : 1;
+            }
+            ++F;
+        }
+    if (st == DGX_OK) {
+        *n_elem = E;
+        *n_face = F;
+        for (int64_t e = 0; e < E; ++e) {
+            if (xyz_g)
+                for (int k = 0; k < d; ++k)
+                    for (int v = 0; v < nvert; ++v)
+                        xyz_g[(e * d + k) * nvert + v] = g.xyz[3 * vn[e * nvert + v] + k];
+            if (node_nums)
+                for (int v = 0; v < nvert; ++v)
+                    node_nums[e * nvert + v] = vn[e * nvert + v];
+            for (int lf = 0; lf < nlf; ++lf) {
+                if (v_to_v)
+                    v_to_v[e * nlf + lf] = vv[e * nlf + lf];
+                if (v_to_lf)
+                    v_to_lf[e * nlf + lf] = vl[e * nlf + lf];
+            }
+        }
+    }
+    free(vn); free(bkeys); free(fk); free(fv_sub); free(bc); free(vv); free(vl);
+    gm_free(&g);
+    return st;
 }

@@ -7,6 +7,7 @@ explicit path is measured on.  All numerical work happens in the C library; this
 numpy buffers.
 """
 from dataclasses import dataclass
+import ctypes
 import math
 
 import numpy as np
@@ -25,6 +26,7 @@ class Mesh:
     face_elem: np.ndarray   # [F][2] int32
     face_lf: np.ndarray     # [F][2] int8
     face_ord: np.ndarray    # [F] int8
+    h: float = 0.0          # smallest element edge when the mesh is not a structured box (0: from n, lo, hi)
 
     @property
     def n_elem(self):
@@ -36,6 +38,8 @@ class Mesh:
 
     @property
     def h_min(self):
+        if self.h > 0.0:
+            return self.h
         return min((self.hi[k] - self.lo[k]) / self.n[k] for k in range(self.d))
 
     def c_struct(self):
@@ -84,6 +88,38 @@ def structured_mesh(d, n, lo, hi, p_g=1, amp=0.0, seed=0):
     _ffi.check(lib.dgx_mesh_structured(d, _ffi.i32ptr(na), _ffi.dptr(lo_a), _ffi.dptr(hi_a), p_g, float(amp),
                                        int(seed), _ffi.dptr(xyz), _ffi.i32ptr(fe), _ffi.i8ptr(fl), _ffi.i8ptr(fo)))
     return Mesh(d, p_g, n, tuple(lo), tuple(hi), xyz, fe, fl, fo)
+
+
+def read_gmsh(path, d):
+    """Mesh from a Gmsh 2.2 ASCII file of QUAD (d=2) / HEX (d=3) volumes with periodic boundaries, the
+    reference's mesh input (dgx_mesh_read_gmsh in include/dgx_setup.h).  Returns (mesh, conn) with conn
+    the reference's connectivity {"v_to_v", "v_to_lf", "node_nums"}; boundary faces that are not
+    periodic leave the volume without a face record (dgx_create then rejects the mesh)."""
+    lib = _ffi.load()
+    bpath = str(path).encode()
+    E, F = ctypes.c_int64(0), ctypes.c_int64(0)
+    _ffi.check(lib.dgx_mesh_read_gmsh(bpath, d, ctypes.byref(E), ctypes.byref(F), None, None, None, None, None,
+                                      None, None))
+    E, F = E.value, F.value
+    nv = 2 ** d
+    xyz = np.empty((E, d, nv))
+    fe = np.empty((F, 2), dtype=np.int32)
+    fl = np.empty((F, 2), dtype=np.int8)
+    fo = np.empty((F,), dtype=np.int8)
+    vv = np.empty((E, 2 * d), dtype=np.int32)
+    vl = np.empty((E, 2 * d), dtype=np.int32)
+    nn = np.empty((E, nv), dtype=np.int32)
+    e_, f_ = ctypes.c_int64(0), ctypes.c_int64(0)
+    _ffi.check(lib.dgx_mesh_read_gmsh(bpath, d, ctypes.byref(e_), ctypes.byref(f_), _ffi.dptr(xyz), _ffi.i32ptr(fe),
+                                      _ffi.i8ptr(fl), _ffi.i8ptr(fo), _ffi.i32ptr(vv), _ffi.i32ptr(vl),
+                                      _ffi.i32ptr(nn)))
+    # smallest edge: vertex pairs that differ in one tensor-product bit
+    h = min(float(np.min(np.linalg.norm(xyz[:, :, v | (1 << k)] - xyz[:, :, v], axis=1)))
+            for k in range(d) for v in range(nv) if not v & (1 << k))
+    lo = tuple(float(xyz[:, k].min()) for k in range(d))
+    hi = tuple(float(xyz[:, k].max()) for k in range(d))
+    mesh = Mesh(d, 1, (0,) * d, lo, hi, xyz, fe, fl, fo, h)
+    return mesh, {"v_to_v": vv, "v_to_lf": vl, "node_nums": nn}
 
 
 def geometry(mesh, p):
@@ -152,7 +188,6 @@ def nodes_1d(family, p):
 
 
 def ctypes_byref(o):
-    import ctypes
     return ctypes.byref(o)
 
 

@@ -56,8 +56,10 @@ int64_t dgx_mesh_structured_n_elem(int d, const int* n);
 int64_t dgx_mesh_structured_n_face(int d, const int* n);
 
 /* Structured periodic QUAD (d=2) / HEX (d=3) mesh on the box [lo,hi].
- * Element e = ix + n0*(iy + n1*iz); faces f = dir*n_elem + e join e (left, local face 2*dir+1) to its
- * periodic +dir neighbour (right, local face 2*dir).  Geometry nodes are GLL of order p_g per direction,
+ * Element e = ix + n0*(iy + n1*iz); face f = dir*n_elem + e is the +dir face of e.  Its left volume is
+ * the one met first in volume order, as the reference's face construction takes it (face.c:109-146):
+ * e (local face 2*dir+1) with its +dir neighbour on the right (local face 2*dir), except on the periodic
+ * wrap-around, where the neighbour at index 0 of that direction is left (local face 2*dir) and e right.  Geometry nodes are GLL of order p_g per direction,
  * ordered r fastest.  When amp != 0 every node is displaced by
  *     x_k += amp * h_k * prod_{m != k} sin(x_m + phi[k][m]),   phi[k][m] = 2*pi*U(seed)
  * evaluated at the undisplaced position (integer wavenumber, so the curving is periodic on [0,2pi]^d).
@@ -65,6 +67,25 @@ int64_t dgx_mesh_structured_n_face(int d, const int* n);
 dgx_status dgx_mesh_structured(int d, const int* n, const double* lo, const double* hi, int p_g,
                                double amp, uint64_t seed,
                                double* xyz_g, int32_t* face_elem, int8_t* face_lf, int8_t* face_ord);
+
+/* Mesh ingest in the reference's on-disk format: a Gmsh 2.2 ASCII file of QUAD (d=2) or HEX (d=3)
+ * volumes with its $Periodic section (mesh_readers.c:56-440, reorder_nodes_gmsh :132-162;
+ * mesh_connectivity.c:93-127,285-393; mesh_periodic.c:102-140).  Volumes are numbered in file order,
+ * their vertices put in tensor-product order (Gmsh QUAD 0 1 2 3 -> 0 1 3 2, HEX -> 0 1 3 2 4 5 7 6) and
+ * used as p_g = 1 geometry nodes.  Local faces lf = 2 dn + side are matched by their sorted vertices, the
+ * vertices of a face on a periodic slave entity replaced by the master's ($Periodic node pairs).
+ * Faces are numbered in the order the volumes (then their local faces) meet them; the first volume is
+ * the face's left volume.  face_ord = 0 when both volumes list the face's vertices in the same order.
+ * Call with null array pointers to get n_elem / n_face, then with arrays of
+ *   xyz_g[n_elem][d][2^d], face_elem[n_face][2], face_lf[n_face][2], face_ord[n_face]
+ * (all four or none) and, nullable, the reference's connectivity for checking:
+ *   v_to_v[n_elem][2d]   neighbour volume of each local face, -1 on a boundary
+ *   v_to_lf[n_elem][2d]  the neighbour's local face, or the boundary face's physical tag
+ *   node_nums[n_elem][2^d] vertex (node index in file order) of each volume, tensor-product order.
+ * DGX_ERR_INVALID_ARG for unreadable files, other volume types, or a face shared by three volumes. */
+dgx_status dgx_mesh_read_gmsh(const char* path, int d, int64_t* n_elem, int64_t* n_face, double* xyz_g,
+                              int32_t* face_elem, int8_t* face_lf, int8_t* face_ord, int32_t* v_to_v,
+                              int32_t* v_to_lf, int32_t* node_nums);
 
 /* Geometry factors for solution order p on GL cubature (collocated GL/GL), following
  * geometry_T.c:734-804 (metric terms at the GLL metric nodes, interpolated to the cubature nodes),

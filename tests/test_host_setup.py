@@ -82,8 +82,12 @@ def test_structured_mesh_connectivity(d, n):
             key = (int(mesh.face_elem[f, side]), int(mesh.face_lf[f, side]))
             assert key not in slots
             slots.add(key)
-        # left face is the + face of its direction, right face the - face of the same direction
-        assert mesh.face_lf[f, 0] == mesh.face_lf[f, 1] + 1 and mesh.face_lf[f, 1] % 2 == 0
+        # the two sides are opposite faces of one direction; the left volume is the one met first in
+        # volume order (the reference's face construction): lower index, the same volume's - face first
+        assert mesh.face_lf[f, 0] // 2 == mesh.face_lf[f, 1] // 2
+        assert mesh.face_lf[f, 0] % 2 != mesh.face_lf[f, 1] % 2
+        eL, eR = mesh.face_elem[f]
+        assert eL < eR or (eL == eR and mesh.face_lf[f, 0] < mesh.face_lf[f, 1])
     assert len(slots) == 2 * d * E
     # shared faces coincide geometrically (up to the periodic shift)
     ng = 3
@@ -92,8 +96,8 @@ def test_structured_mesh_connectivity(d, n):
         eL, eR = mesh.face_elem[f]
         dn = mesh.face_lf[f, 0] // 2
         axis = d - dn          # array axis of direction dn in [E][d][t][s][r] order
-        a = np.take(xg[eL], ng - 1, axis=axis)
-        b = np.take(xg[eR], 0, axis=axis)
+        a = np.take(xg[eL], ng - 1 if mesh.face_lf[f, 0] % 2 else 0, axis=axis)
+        b = np.take(xg[eR], ng - 1 if mesh.face_lf[f, 1] % 2 else 0, axis=axis)
         diff = a - b
         diff[dn] = np.mod(diff[dn] + L / 2, L) - L / 2
         assert np.abs(diff).max() < 1e-12
