@@ -2,13 +2,16 @@
 SSP-RK3 stage with positivity) of DPGSolver, behind the C-ABI in include/dgx.h.
 
 The compute path is libdgx.so (hand-written gfx950 kernels); this package only holds ctypes bindings,
-the host-side set-up helpers and the solver mirror.
+the host-side set-up helpers, the solver mirror and the matrix-free implicit solver over the
+linearized-residual kernels.
 """
 from . import _ffi
-from .problem import (Geometry, Mesh, dt_cfl, geometry, ic_isentropic_vortex, ic_taylor_green, randn,
-                      read_gmsh, solution_nodes_xyz, structured_mesh, tgv_3d, vortex_2d)
+from .problem import (Geometry, Mesh, dt_cfl, geometry, ic_isentropic_vortex, ic_taylor_green, ic_uniform_flow,
+                      randn, read_gmsh, solution_nodes_xyz, structured_mesh, tgv_3d, vortex_2d)
 from .solver import Solver, residual_function
+from .implicit import BlockJacobi, gmres, solve_implicit, solve_linear
 
 __all__ = ["Mesh", "Geometry", "Solver", "structured_mesh", "geometry", "ic_isentropic_vortex",
            "ic_taylor_green", "dt_cfl", "randn", "solution_nodes_xyz", "vortex_2d", "tgv_3d",
-           "residual_function", "read_gmsh"]
+           "residual_function", "read_gmsh", "ic_uniform_flow", "solve_implicit", "solve_linear", "gmres",
+           "BlockJacobi"]

@@ -1,0 +1,275 @@
+"""Matrix-free implicit solver: pseudo-transient Newton-Krylov on the explicit path's residual.
+
+Mirrors the reference's implicit DG loop (solve_implicit.c:95-110; implicit_step :232-256; check_exit
+:258-284; solve_and_update / update_coefs :393-468) with its CFL-ramped pseudo-time term
+(compute_CFL_ramping and compute_dt_cfl_constrained, dg/solve_dg.c:222-318), and replaces the assembled
+PETSc matrix and KSP by restarted GMRES on Jacobian-vector products of the residual (dgx_jvp, one kernel
+launch per Krylov iteration; the state never leaves HBM).  Each step solves
+
+    (M/dt - dR/du) du = R(u)   <=>   (I/dt - M^-1 dR/du) du = M^-1 R(u),    u <- u + du,
+
+with dt per volume = cfl * h_v / max(|V| + c) over the volume's Bezier coefficients and
+cfl = cfl_initial * max(1, max_rhs_0 / max_rhs).  The reference preconditions with ILU(1)
+(external/petsc/options_files/petsc_options_gmres_default.txt); a matrix-free solver has no assembled
+matrix to factor, so the preconditioner here is block Jacobi on the volume-diagonal blocks, which are
+probed with JVPs on a colouring of the volumes and inverted in one batched call.
+
+Transposed systems ((I/dt - M^-1 dR/du)^T psi = g, for output adjoints) use dgx_vjp and the transposed
+blocks.  Euler only: dgx_jvp / dgx_vjp refuse Navier-Stokes contexts.
+"""
+import math
+
+import numpy as np
+
+GAMMA = 1.4
+
+
+def gmres(matvec, b, rtol=1e-4, restart=60, max_iter=600):
+    """Restarted GMRES (Saad & Schultz 1986) with Givens rotations; Arnoldi by classical Gram-Schmidt
+    applied twice (as stable as the modified form, and two matrix-vector products over the basis
+    instead of k dot products, so one host synchronisation per iteration).
+
+    matvec maps a flat float64 tensor to one of the same shape (torch, CPU or GPU); b is the right-hand
+    side.  Returns (x, relative residual ||b - A x|| / ||b||, iterations)."""
+    import torch
+    x = torch.zeros_like(b)
+    bnorm = float(torch.linalg.norm(b))
+    if bnorm == 0.0:
+        return x, 0.0, 0
+    V = torch.empty((restart + 1, b.numel()), dtype=b.dtype, device=b.device)
+    r = b
+    beta = bnorm
+    it = 0
+    while it < max_iter:
+        V[0] = r / beta
+        H = np.zeros((restart + 1, restart))
+        cs, sn = np.zeros(restart), np.zeros(restart)
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        k_used = 0
+        for k in range(restart):
+            w = matvec(V[k])
+            Vk = V[:k + 1]
+            h = torch.mv(Vk, w)
+            w = w - torch.mv(Vk.t(), h)
+            h2 = torch.mv(Vk, w)
+            w = w - torch.mv(Vk.t(), h2)
+            hk1 = torch.linalg.norm(w)
+            col = torch.cat([h + h2, hk1.view(1)]).cpu().numpy()
+            H[:k + 2, k] = col
+            hk1 = float(col[-1])
+            for j in range(k):
+                t = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                H[j, k] = t
+            den = math.hypot(H[k, k], H[k + 1, k])
+            cs[k], sn[k] = (H[k, k] / den, H[k + 1, k] / den) if den > 0.0 else (1.0, 0.0)
+            H[k, k], H[k + 1, k] = den, 0.0
+            g[k + 1] = -sn[k] * g[k]
+            g[k] = cs[k] * g[k]
+            it += 1
+            k_used = k + 1
+            if abs(g[k + 1]) <= rtol * bnorm or it >= max_iter or hk1 == 0.0:
+                break
+            V[k + 1] = w / hk1
+        y = np.zeros(k_used)
+        for i in range(k_used - 1, -1, -1):
+            if H[i, i] != 0.0:
+                y[i] = (g[i] - H[i, i + 1:k_used] @ y[i + 1:k_used]) / H[i, i]
+        x = x + torch.mv(V[:k_used].t(), torch.from_numpy(y).to(b.device))
+        r = b - matvec(x)
+        beta = float(torch.linalg.norm(r))
+        if beta <= rtol * bnorm or beta == 0.0:
+            break
+    return x, beta / bnorm, it
+
+
+def element_colors(mesh):
+    """Greedy distance-1 colouring of the volumes over the face list: volumes that share a face get
+    different colours.  Returns (colour per volume, number of colours)."""
+    E = mesh.n_elem
+    nbrs = [set() for _ in range(E)]
+    for l, r in np.asarray(mesh.face_elem).reshape(-1, 2):
+        if l == r:
+            raise ValueError(f"volume {l} is its own face neighbour; no block colouring exists")
+        nbrs[l].add(int(r))
+        nbrs[r].add(int(l))
+    col = np.full(E, -1, dtype=np.int64)
+    for e in range(E):
+        used = {col[n] for n in nbrs[e]}
+        c = 0
+        while c in used:
+            c += 1
+        col[e] = c
+    return col, int(col.max()) + 1
+
+
+def block_diagonal(solver, colors, n_colors, device):
+    """Volume-diagonal blocks of J = d(M^-1 R)/du at the solver's current state, [E][nv*Ns][nv*Ns]
+    (row and column index v*Ns + node), by probing: one JVP per (colour, variable, node) with unit
+    entries on all volumes of that colour, no two of which share a face, so every volume's block column
+    arrives free of its neighbours' contributions."""
+    import torch
+    E, nv, ns = solver.shape
+    n = nv * ns
+    col = torch.as_tensor(colors, device=device)
+    B = torch.zeros((E, n, n), dtype=torch.float64, device=device)
+    w = torch.zeros(solver.shape, dtype=torch.float64, device=device)
+    jw = torch.empty_like(w)
+    sync = _sync_fn(device)
+    for c in range(n_colors):
+        sel = col == c
+        for j in range(n):
+            v, k = divmod(j, ns)
+            w.zero_()
+            w[sel, v, k] = 1.0
+            sync()
+            solver.jvp(w, apply_minv=True, out=jw)
+            B[sel, :, j] = jw[sel].reshape(-1, n)
+    return B
+
+
+class BlockJacobi:
+    """Block-Jacobi preconditioner of A = diag(inv_dt) - M^-1 dR/du on the volume-diagonal blocks."""
+
+    def __init__(self, solver, mesh, device="cuda"):
+        self.solver, self.device = solver, device
+        self.colors, self.n_colors = element_colors(mesh)
+        self.Pinv = None
+
+    def update(self, inv_dt):
+        """Probe the blocks at the solver's current state and invert diag(inv_dt) - B per volume."""
+        import torch
+        E, nv, ns = self.solver.shape
+        B = block_diagonal(self.solver, self.colors, self.n_colors, self.device)
+        eye = torch.eye(nv * ns, dtype=torch.float64, device=self.device)
+        idt = torch.as_tensor(inv_dt, dtype=torch.float64, device=self.device)
+        idt = idt.view(-1, 1, 1) if idt.dim() else idt
+        self.Pinv = torch.linalg.inv(eye * idt - B)
+
+    def apply(self, y, transpose=False):
+        import torch
+        E, n = self.Pinv.shape[0], self.Pinv.shape[1]
+        P = self.Pinv.transpose(1, 2).contiguous() if transpose else self.Pinv
+        return torch.bmm(P, y.reshape(E, n, 1)).reshape(-1)
+
+
+def solve_linear(solver, b, inv_dt=0.0, transpose=False, precond=None, rtol=1e-4, restart=60, max_iter=600):
+    """Solve (diag(inv_dt) - M^-1 dR/du) x = b, or its transpose, at the solver's current state by
+    right-preconditioned GMRES.  b: torch float64 tensor [E][nv][Ns] on the solver's device; inv_dt: a
+    scalar or one value per volume.  Returns (x, relative residual, iterations)."""
+    import torch
+    shape = solver.shape
+    sync = _sync_fn(b.device)
+    out = torch.empty(shape, dtype=torch.float64, device=b.device)
+    idt = torch.as_tensor(inv_dt, dtype=torch.float64, device=b.device)
+    idt = idt.view(-1, 1, 1) if idt.dim() else idt
+    op = solver.vjp if transpose else solver.jvp
+
+    def apply_a(w):
+        wv = w.view(shape)
+        sync()
+        op(wv, apply_minv=True, out=out)
+        return (wv * idt - out).reshape(-1)
+
+    def prec(y):
+        return precond.apply(y, transpose) if precond is not None else y
+
+    y, rel, its = gmres(lambda y: apply_a(prec(y)), b.reshape(-1).clone(), rtol=rtol, restart=restart,
+                        max_iter=max_iter)
+    return prec(y).view(shape), rel, its
+
+
+def volume_lengths(mesh):
+    """Smallest edge length of every volume from its corner geometry nodes (compute_min_length_measure,
+    dg/solve_dg.c:320-380: the minimum over the volume's edges for d >= 2)."""
+    d, ng = mesh.d, mesh.p_g + 1
+    xg = mesh.xyz_g.reshape(mesh.n_elem, d, *([ng] * d))      # [E][d][(t)][s][r]: r runs fastest
+    h = np.full(mesh.n_elem, np.inf)
+    for c in range(1 << d):
+        a = [(c >> m) & 1 for m in range(d)]                 # corner bits per reference direction r, s, t
+        for k in range(d):
+            if a[k]:
+                continue
+            b = list(a)
+            b[k] = 1
+            ia = tuple((ng - 1) * a[d - 1 - m] for m in range(d))
+            ib = tuple((ng - 1) * b[d - 1 - m] for m in range(d))
+            ea = xg[(slice(None), slice(None)) + ia]
+            eb = xg[(slice(None), slice(None)) + ib]
+            h = np.minimum(h, np.linalg.norm(ea - eb, axis=1))
+    return h
+
+
+def max_wave_speed(u, sb, d):
+    """Per volume, the maximum of |V| + c over the Bezier coefficients of the conserved variables
+    (compute_dt_cfl_constrained, dg/solve_dg.c:283-297; compute_max_wavespeed,
+    test_case/solution/euler/solution_euler.c:143-170).  u: [E][nv][Ns]; sb: nodal -> Bezier (n1, n1)."""
+    import torch
+    E, nv, ns = u.shape
+    n1 = sb.shape[0]
+    x = u.reshape(E, nv, *([n1] * d))
+    sbt = sb.t().contiguous()
+    for ax in range(d):
+        x = torch.movedim(torch.movedim(x, -1 - ax, -1) @ sbt, -1, -1 - ax)
+    x = x.reshape(E, nv, ns)
+    rho = x[:, 0]
+    vel2 = (x[:, 1:d + 1] ** 2).sum(dim=1) / rho ** 2
+    pres = (GAMMA - 1.0) * (x[:, d + 1] - 0.5 * rho * vel2)
+    return (vel2.sqrt() + (GAMMA * pres / rho).sqrt()).amax(dim=1)
+
+
+def solve_implicit(solver, mesh, p, cfl_initial=1.0, exit_tol=1e-10, exit_ratio=0.0, max_steps=50,
+                   rtol=1e-4, restart=60, max_iter=600, precondition=True, verbose=False, device="cuda"):
+    """Pseudo-transient Newton-Krylov from the solver's current state until max |M^-1 R| < exit_tol or
+    max_rhs / max_rhs_0 < exit_ratio (check_exit, solve_implicit.c:258-284).
+
+    Defaults follow the reference's GMRES options (restart 60, rtol 1e-4).  Returns (final state
+    [E][nv][Ns] as a torch tensor on `device`, history) with one entry per step: (max_rhs, cfl, GMRES
+    iterations, GMRES relative residual); the last entry holds the final max_rhs.  The solver is left
+    at the final state."""
+    import torch
+    from .problem import ops1d
+    dev = torch.device(device)
+    sync = _sync_fn(dev)
+    shape = solver.shape
+    u = solver.get_state(torch.empty(shape, dtype=torch.float64, device=dev))
+    h = torch.from_numpy(volume_lengths(mesh)).to(dev)
+    sb = torch.from_numpy(ops1d(p)["sb"]).to(dev)
+    precond = BlockJacobi(solver, mesh, dev) if precondition else None
+    r = torch.empty_like(u)
+    hist = []
+    max_rhs0 = None
+    for step in range(max_steps + 1):
+        sync()
+        solver.set_state(u)
+        _, mx = solver.compute_rhs(apply_minv=True, out=r)
+        if not math.isfinite(mx):
+            raise FloatingPointError(f"non-finite residual at implicit step {step}")
+        if max_rhs0 is None:
+            max_rhs0 = mx
+        if mx < exit_tol or mx < exit_ratio * max_rhs0 or step == max_steps:
+            hist.append((mx, 0.0, 0, 0.0))
+            break
+        cfl = cfl_initial * max(1.0, max_rhs0 / mx)
+        inv_dt = max_wave_speed(u, sb, mesh.d) / (cfl * h)
+        bad = torch.nonzero(~torch.isfinite(inv_dt))
+        if bad.numel():
+            raise FloatingPointError(f"implicit step {step}: no wave speed in volume {int(bad[0, 0])} (a Bezier "
+                                     "coefficient with non-positive density or pressure)")
+        if precond is not None:
+            precond.update(inv_dt)
+        du, rel, its = solve_linear(solver, r, inv_dt, precond=precond, rtol=rtol, restart=restart,
+                                    max_iter=max_iter)
+        hist.append((mx, cfl, its, rel))
+        if verbose:
+            print(f"implicit step {step}: max_rhs {mx:.3e} cfl {cfl:.3g} gmres {its} ({rel:.1e})", flush=True)
+        u = u + du
+    sync()
+    solver.set_state(u)
+    return u, hist
+
+
+def _sync_fn(device):
+    import torch
+    return torch.cuda.synchronize if torch.device(device).type == "cuda" else (lambda: None)

@@ -155,6 +155,30 @@ def ic_taylor_green(mesh, p, mach=0.1):
     return u
 
 
+def ic_uniform_flow(mesh, p, mach=2.0, angle=0.5, amp=1e-2, gamma=1.4):
+    """Uniform flow (rho = 1, p = 1/gamma, so c = 1) at speed `mach` in direction (cos a, sin a) (2-D) or
+    (cos a cos a/2, sin a cos a/2, sin a/2) (3-D), plus smooth periodic perturbations of amplitude `amp`
+    on [0, 2 pi]^d, each varying along every coordinate (so that no face sees an exactly continuous
+    normal velocity, where the Roe-Pike eigenvalue selection ties).  The steady state of a periodic domain
+    is a uniform flow; the perturbation gives the implicit solver a residual to drive to zero."""
+    x = solution_nodes_xyz(mesh, p)
+    d = mesh.d
+    X, Y = x[:, 0], x[:, 1]
+    Z = x[:, 2] if d == 3 else 0.0
+    if d == 2:
+        vdir = (math.cos(angle), math.sin(angle))
+    else:
+        vdir = (math.cos(angle) * math.cos(0.5 * angle), math.sin(angle) * math.cos(0.5 * angle),
+                math.sin(0.5 * angle))
+    rho = 1.0 + amp * np.sin(X + 2 * Y + 3 * Z)
+    pr = 1.0 / gamma + amp * np.cos(2 * X + Y - Z)
+    vel = [mach * vdir[0] + amp * np.sin(2 * X - Y + Z), mach * vdir[1] + amp * np.cos(X + 3 * Y - 2 * Z)]
+    if d == 3:
+        vel.append(mach * vdir[2] + amp * np.sin(3 * X + Y + 2 * Z))
+    ke = 0.5 * rho * sum(v * v for v in vel)
+    return np.stack([rho] + [rho * v for v in vel] + [pr / (gamma - 1.0) + ke], axis=1)
+
+
 def dt_cfl(mesh, p, u, cfl=0.5):
     lib = _ffi.load()
     u = np.ascontiguousarray(u, dtype=np.float64)
