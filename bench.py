@@ -182,12 +182,133 @@ def bench_vjp(args, dg, torch):
           flush=True)
 
 
+def bench_implicit(args, dg, torch):
+    """Pseudo-transient Newton-Krylov (dpgsolver_amd.implicit) on a 3-D Euler HEX p=3 n^3 periodic mesh,
+    perturbed M=2 uniform flow, Roe-Pike: `--newton-steps` Newton steps from the same initial state, each
+    = block probing (n_colours * 320 JVPs) + batched inversion + GMRES(60) to rtol 1e-4 (at most 600
+    iterations).  One unit = one GMRES iteration (one JVP, one preconditioner application, CGS2 over the
+    basis)."""
+    from dpgsolver_amd import implicit
+    n, p = args.nk_n, 3
+    mesh = dg.structured_mesh(3, (n, n, n), (0, 0, 0), (2 * math.pi,) * 3, p_g=1)
+    geo = dg.geometry(mesh, p)
+    u0 = dg.ic_uniform_flow(mesh, p, amp=1e-2)
+    s = dg.Solver(mesh, geo, p, num_flux="roe")
+
+    class Timed:
+        """The solver, with the JVP's HIP-event time (layout copy, traces, lin kernel) accumulated."""
+
+        def __init__(self, inner):
+            self.inner, self.shape = inner, inner.shape
+            self.jvp_ms, self.jvp_calls = 0.0, 0
+
+        def __getattr__(self, name):
+            return getattr(self.inner, name)
+
+        def jvp(self, w, apply_minv=False, out=None):
+            r = self.inner.jvp(w, apply_minv=apply_minv, out=out)
+            self.jvp_ms += self.inner.last_elapsed_ms
+            self.jvp_calls += 1
+            return r
+
+    class TimedPrec(implicit.BlockJacobi):
+        def __init__(self, *a, **k):
+            super().__init__(*a, **k)
+            self.update_s, self.apply_s, self.applies = 0.0, 0.0, 0
+
+        def update(self, inv_dt):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            super().update(inv_dt)
+            torch.cuda.synchronize()
+            self.update_s += time.perf_counter() - t0
+
+        def apply(self, y, transpose=False):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = super().apply(y, transpose)
+            torch.cuda.synchronize()
+            self.apply_s += time.perf_counter() - t0
+            self.applies += 1
+            return r
+
+    def run(steps, timed):
+        s.set_state(torch.from_numpy(u0).cuda())
+        pc = TimedPrec(timed, mesh, "cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hist = implicit.solve_implicit(timed, mesh, p, exit_tol=0.0, max_steps=steps, precond=pc)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, hist, pc
+
+    run(1, Timed(s))                                   # warm-up: allocations, library kernels
+    timed = Timed(s)
+    wall, hist, pc = run(args.newton_steps, timed)
+    its = sum(h[2] for h in hist)
+    probes = pc.n_colors * s.nv * s.ns * args.newton_steps
+    gmres_jvps = timed.jvp_calls - probes
+    jvp_ms = timed.jvp_ms / max(timed.jvp_calls, 1)
+    b_elem = algorithmic_bytes_per_element(3, p)
+    achieved = b_elem * mesh.n_elem / (jvp_ms * 1e-3) / 1e9
+    gmres_s = wall - pc.update_s
+    dof = mesh.n_elem * s.ns * s.nv
+    blk_bytes = mesh.n_elem * (s.nv * s.ns) ** 2 * 4
+    apply_ms = pc.apply_s / max(pc.applies, 1) * 1e3
+    out = {"metric": "GMRES iterations/s of the matrix-free Newton-Krylov solve, HEX p=3 Euler",
+           "value": its / gmres_s, "unit": "GMRES iterations/s", "n_gpus": 1, "steps": args.newton_steps,
+           "warmup": 1, "ms_per_step": wall / args.newton_steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64 (f32 preconditioner blocks)",
+           "data": "synthetic",
+           "config": {"workload": f"implicit Newton-GMRES {n}^3 HEX p=3 periodic [0,2pi]^3, M=2 uniform flow + "
+                                  f"1e-2 perturbation, Roe-Pike, block-Jacobi (f32), GMRES(60) rtol 1e-4",
+                      "elements": mesh.n_elem, "dof": dof, "newton_steps": args.newton_steps,
+                      "gmres_iterations": its, "max_rhs": [h[0] for h in hist]},
+           "breakdown_ms": {"newton_step": wall / args.newton_steps * 1e3,
+                            "precond_update_per_step": pc.update_s / args.newton_steps * 1e3,
+                            "gmres_per_iteration": gmres_s / max(its, 1) * 1e3,
+                            "jvp_per_call": jvp_ms, "precond_apply_per_call": apply_ms,
+                            "jvp_calls": timed.jvp_calls, "gmres_jvp_calls": gmres_jvps},
+           "precond_apply": {"bytes": blk_bytes, "achieved": blk_bytes / (apply_ms * 1e-3) / 1e9,
+                             "unit": "GB/s", "kernel": "torch.bmm f32 (batched GEMV)"},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "dgx::lin_kernel<3,3,1,false> (+ layout and trace kernels)"}}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_jvp(args, dg, mesh, geo, u0, p, timed)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_jvp(args, dg, mesh, geo, u0, p, timed):
+    """The oracle's complex-step JVP of M^-1 R (the CPU restatement's linearization) on the same mesh
+    and state, on all host threads, for about --cpu-seconds; unit JVP/s."""
+    from oracle.oracle import Oracle
+    orc = Oracle(mesh, p, num_flux=1, positivity=False, geometry=geo)
+    nt = host_threads()
+    orc.set_threads(nt)
+    w = dg.randn(5, u0.shape)
+    calls = 0
+    t0 = time.perf_counter()
+    while True:
+        orc.jvp_cs(u0, w, apply_minv=True)
+        calls += 1
+        el = time.perf_counter() - t0
+        if el > args.cpu_seconds:
+            break
+    gpu = timed.jvp_calls / (timed.jvp_ms * 1e-3) if timed.jvp_ms > 0 else None
+    return {"value": calls / el, "unit": "JVP/s", "cores": nt, "kind": "port",
+            "sample": f"{calls} complex-step JVPs of M^-1 R in {el:.1f} s on {nt} threads, same {mesh.n_elem}-element "
+                      f"mesh and state ({cpu_model()})",
+            "gpu_jvp_per_s": gpu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="hex64", choices=["hex64", "hex32", "vortex2d", "ns48", "vjp32"])
+    ap.add_argument("--config", default="hex64", choices=["hex64", "hex32", "vortex2d", "ns48", "vjp32", "nk"])
+    ap.add_argument("--nk-n", type=int, default=16)
+    ap.add_argument("--newton-steps", type=int, default=4)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -208,6 +329,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+
+    if args.config == "nk":
+        return bench_implicit(args, dg, torch)
 
     t_setup = time.perf_counter()
     viscous = None

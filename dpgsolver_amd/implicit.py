@@ -24,10 +24,13 @@ import numpy as np
 GAMMA = 1.4
 
 
-def gmres(matvec, b, rtol=1e-4, restart=60, max_iter=600):
+def gmres(matvec, b, rtol=1e-4, restart=60, max_iter=600, precond=None):
     """Restarted GMRES (Saad & Schultz 1986) with Givens rotations; Arnoldi by classical Gram-Schmidt
     applied twice (as stable as the modified form, and two matrix-vector products over the basis
-    instead of k dot products, so one host synchronisation per iteration).
+    instead of k dot products, so one host synchronisation per iteration).  With `precond`, the
+    flexible right-preconditioned form (Saad 1993): the preconditioned vectors z_k are kept and the
+    update is x += Z y, so a preconditioner applied in lower precision (not exactly linear) does not
+    limit the attainable residual.
 
     matvec maps a flat float64 tensor to one of the same shape (torch, CPU or GPU); b is the right-hand
     side.  Returns (x, relative residual ||b - A x|| / ||b||, iterations)."""
@@ -37,6 +40,7 @@ def gmres(matvec, b, rtol=1e-4, restart=60, max_iter=600):
     if bnorm == 0.0:
         return x, 0.0, 0
     V = torch.empty((restart + 1, b.numel()), dtype=b.dtype, device=b.device)
+    Z = torch.empty((restart, b.numel()), dtype=b.dtype, device=b.device) if precond is not None else V
     r = b
     beta = bnorm
     it = 0
@@ -48,7 +52,9 @@ def gmres(matvec, b, rtol=1e-4, restart=60, max_iter=600):
         g[0] = beta
         k_used = 0
         for k in range(restart):
-            w = matvec(V[k])
+            if precond is not None:
+                Z[k] = precond(V[k])
+            w = matvec(Z[k])
             Vk = V[:k + 1]
             h = torch.mv(Vk, w)
             w = w - torch.mv(Vk.t(), h)
@@ -76,7 +82,7 @@ def gmres(matvec, b, rtol=1e-4, restart=60, max_iter=600):
         for i in range(k_used - 1, -1, -1):
             if H[i, i] != 0.0:
                 y[i] = (g[i] - H[i, i + 1:k_used] @ y[i + 1:k_used]) / H[i, i]
-        x = x + torch.mv(V[:k_used].t(), torch.from_numpy(y).to(b.device))
+        x = x + torch.mv(Z[:k_used].t(), torch.from_numpy(y).to(b.device))
         r = b - matvec(x)
         beta = float(torch.linalg.norm(r))
         if beta <= rtol * bnorm or beta == 0.0:
@@ -130,10 +136,17 @@ def block_diagonal(solver, colors, n_colors, device):
 
 
 class BlockJacobi:
-    """Block-Jacobi preconditioner of A = diag(inv_dt) - M^-1 dR/du on the volume-diagonal blocks."""
+    """Block-Jacobi preconditioner of A = diag(inv_dt) - M^-1 dR/du on the volume-diagonal blocks.
 
-    def __init__(self, solver, mesh, device="cuda"):
+    The inverted blocks are kept in `dtype` (float32 by default): applying them is a batched GEMV that
+    reads every block once per Krylov iteration, the largest HBM stream of the iteration, and the
+    preconditioner's precision does not limit the solution's (GMRES measures its residual on A itself,
+    with the FP64 JVP)."""
+
+    def __init__(self, solver, mesh, device="cuda", dtype=None):
+        import torch
         self.solver, self.device = solver, device
+        self.dtype = torch.float32 if dtype is None else dtype
         self.colors, self.n_colors = element_colors(mesh)
         self.Pinv = None
 
@@ -142,21 +155,31 @@ class BlockJacobi:
         import torch
         E, nv, ns = self.solver.shape
         B = block_diagonal(self.solver, self.colors, self.n_colors, self.device)
-        eye = torch.eye(nv * ns, dtype=torch.float64, device=self.device)
-        idt = torch.as_tensor(inv_dt, dtype=torch.float64, device=self.device)
-        idt = idt.view(-1, 1, 1) if idt.dim() else idt
-        self.Pinv = torch.linalg.inv(eye * idt - B)
+        idt = torch.as_tensor(inv_dt, dtype=torch.float64, device=self.device).reshape(-1)
+        B.neg_()
+        B.diagonal(dim1=1, dim2=2).add_(idt.view(-1, 1) if idt.numel() > 1 else idt)
+        # batches of 64 blocks: the batched LU of the solver library fails to allocate its workspace
+        # from 128 blocks of 320 x 320 up (measured on MI355X with ROCm 7)
+        self.Pinv = torch.empty(B.shape, dtype=self.dtype, device=self.device)
+        for i in range(0, E, 64):
+            self.Pinv[i:i + 64] = torch.linalg.inv(B[i:i + 64])
+        self.PinvT = None
 
     def apply(self, y, transpose=False):
         import torch
         E, n = self.Pinv.shape[0], self.Pinv.shape[1]
-        P = self.Pinv.transpose(1, 2).contiguous() if transpose else self.Pinv
-        return torch.bmm(P, y.reshape(E, n, 1)).reshape(-1)
+        if transpose:
+            if self.PinvT is None:
+                self.PinvT = self.Pinv.transpose(1, 2).contiguous()
+            P = self.PinvT
+        else:
+            P = self.Pinv
+        return torch.bmm(P, y.reshape(E, n, 1).to(self.dtype)).reshape(-1).to(y.dtype)
 
 
 def solve_linear(solver, b, inv_dt=0.0, transpose=False, precond=None, rtol=1e-4, restart=60, max_iter=600):
     """Solve (diag(inv_dt) - M^-1 dR/du) x = b, or its transpose, at the solver's current state by
-    right-preconditioned GMRES.  b: torch float64 tensor [E][nv][Ns] on the solver's device; inv_dt: a
+    flexible right-preconditioned GMRES.  b: torch float64 tensor [E][nv][Ns] on the solver's device; inv_dt: a
     scalar or one value per volume.  Returns (x, relative residual, iterations)."""
     import torch
     shape = solver.shape
@@ -172,12 +195,10 @@ def solve_linear(solver, b, inv_dt=0.0, transpose=False, precond=None, rtol=1e-4
         op(wv, apply_minv=True, out=out)
         return (wv * idt - out).reshape(-1)
 
-    def prec(y):
-        return precond.apply(y, transpose) if precond is not None else y
-
-    y, rel, its = gmres(lambda y: apply_a(prec(y)), b.reshape(-1).clone(), rtol=rtol, restart=restart,
-                        max_iter=max_iter)
-    return prec(y).view(shape), rel, its
+    prec = (lambda y: precond.apply(y, transpose)) if precond is not None else None
+    x, rel, its = gmres(apply_a, b.reshape(-1).clone(), rtol=rtol, restart=restart, max_iter=max_iter,
+                        precond=prec)
+    return x.view(shape), rel, its
 
 
 def volume_lengths(mesh):
@@ -220,14 +241,16 @@ def max_wave_speed(u, sb, d):
 
 
 def solve_implicit(solver, mesh, p, cfl_initial=1.0, exit_tol=1e-10, exit_ratio=0.0, max_steps=50,
-                   rtol=1e-4, restart=60, max_iter=600, precondition=True, verbose=False, device="cuda"):
+                   rtol=1e-4, restart=60, max_iter=600, precondition=True, verbose=False, device="cuda",
+                   precond=None):
     """Pseudo-transient Newton-Krylov from the solver's current state until max |M^-1 R| < exit_tol or
     max_rhs / max_rhs_0 < exit_ratio (check_exit, solve_implicit.c:258-284).
 
     Defaults follow the reference's GMRES options (restart 60, rtol 1e-4).  Returns (final state
     [E][nv][Ns] as a torch tensor on `device`, history) with one entry per step: (max_rhs, cfl, GMRES
     iterations, GMRES relative residual); the last entry holds the final max_rhs.  The solver is left
-    at the final state."""
+    at the final state.  `precond`: a BlockJacobi (or an object with its update/apply) to use instead of
+    a new one; precondition=False runs plain GMRES."""
     import torch
     from .problem import ops1d
     dev = torch.device(device)
@@ -236,7 +259,10 @@ def solve_implicit(solver, mesh, p, cfl_initial=1.0, exit_tol=1e-10, exit_ratio=
     u = solver.get_state(torch.empty(shape, dtype=torch.float64, device=dev))
     h = torch.from_numpy(volume_lengths(mesh)).to(dev)
     sb = torch.from_numpy(ops1d(p)["sb"]).to(dev)
-    precond = BlockJacobi(solver, mesh, dev) if precondition else None
+    if not precondition:
+        precond = None
+    elif precond is None:
+        precond = BlockJacobi(solver, mesh, dev)
     r = torch.empty_like(u)
     hist = []
     max_rhs0 = None
