@@ -17,6 +17,7 @@ c_int8_p = ctypes.POINTER(ctypes.c_int8)
 c_int64_p = ctypes.POINTER(ctypes.c_int64)
 
 DGX_OK = 0
+DGX_ERR_INVALID_ARG = 1
 STATUS_NAMES = {
     0: "DGX_OK",
     1: "DGX_ERR_INVALID_ARG",
@@ -69,6 +70,8 @@ SIGNATURES = {
     "dgx_create": (ctypes.c_int, [ctypes.POINTER(DgxConfig), ctypes.POINTER(DgxMesh),
                                   ctypes.POINTER(DgxGeometry), ctypes.POINTER(ctypes.c_void_p)]),
     "dgx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "dgx_block_inverse": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dgx_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "dgx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "dgx_get_stream": (ctypes.c_void_p, [ctypes.c_void_p]),

@@ -118,20 +118,19 @@ def block_diagonal(solver, colors, n_colors, device):
     import torch
     E, nv, ns = solver.shape
     n = nv * ns
-    col = torch.as_tensor(colors, device=device)
-    B = torch.zeros((E, n, n), dtype=torch.float64, device=device)
+    B = torch.empty((E, n, n), dtype=torch.float64, device=device)
     w = torch.zeros(solver.shape, dtype=torch.float64, device=device)
+    wf = w.view(E, n)
     jw = torch.empty_like(w)
     sync = _sync_fn(device)
     for c in range(n_colors):
-        sel = col == c
+        idx = torch.from_numpy(np.nonzero(np.asarray(colors) == c)[0]).to(device)
         for j in range(n):
-            v, k = divmod(j, ns)
-            w.zero_()
-            w[sel, v, k] = 1.0
+            wf[idx, j] = 1.0
             sync()
             solver.jvp(w, apply_minv=True, out=jw)
-            B[sel, :, j] = jw[sel].reshape(-1, n)
+            wf[idx, j] = 0.0
+            B[idx, :, j] = jw.view(E, n).index_select(0, idx)
     return B
 
 
@@ -156,13 +155,18 @@ class BlockJacobi:
         E, nv, ns = self.solver.shape
         B = block_diagonal(self.solver, self.colors, self.n_colors, self.device)
         idt = torch.as_tensor(inv_dt, dtype=torch.float64, device=self.device).reshape(-1)
-        B.neg_()
-        B.diagonal(dim1=1, dim2=2).add_(idt.view(-1, 1) if idt.numel() > 1 else idt)
-        # batches of 64 blocks: the batched LU of the solver library fails to allocate its workspace
-        # from 128 blocks of 320 x 320 up (measured on MI355X with ROCm 7)
-        self.Pinv = torch.empty(B.shape, dtype=self.dtype, device=self.device)
-        for i in range(0, E, 64):
-            self.Pinv[i:i + 64] = torch.linalg.inv(B[i:i + 64])
+        n = nv * ns
+        if B.is_cuda and n <= 320 and self.dtype == torch.float32:
+            self.Pinv = block_inverse(B, idt.expand(E).contiguous())
+        else:
+            # larger blocks (HEX p=4), other precisions and the CPU: the solver library, in batches of
+            # 64 blocks (its batched LU fails to allocate its workspace from 128 blocks of 320 x 320 up
+            # on MI355X with ROCm 7)
+            B.neg_()
+            B.diagonal(dim1=1, dim2=2).add_(idt.view(-1, 1) if idt.numel() > 1 else idt)
+            self.Pinv = torch.empty(B.shape, dtype=self.dtype, device=self.device)
+            for i in range(0, E, 64):
+                self.Pinv[i:i + 64] = torch.linalg.inv(B[i:i + 64])
         self.PinvT = None
 
     def apply(self, y, transpose=False):
@@ -175,6 +179,32 @@ class BlockJacobi:
         else:
             P = self.Pinv
         return torch.bmm(P, y.reshape(E, n, 1).to(self.dtype)).reshape(-1).to(y.dtype)
+
+
+def block_inverse(B, shift):
+    """(shift_b I - B_b)^-1 for every block on the GPU (dgx_block_inverse: FP32 Gauss-Jordan with partial
+    pivoting, one workgroup per block, n <= 320).  B: float64 cuda tensor [nb][n][n]; shift: float64
+    cuda tensor [nb].  Returns float32 [nb][n][n]; raises if a block is singular."""
+    import ctypes
+    import torch
+    from . import _ffi
+    nb, n = B.shape[0], B.shape[1]
+    if not (B.is_cuda and B.dtype == torch.float64 and B.is_contiguous() and B.shape == (nb, n, n)):
+        raise ValueError("B must be a contiguous float64 cuda tensor [nb][n][n]")
+    if not (shift.is_cuda and shift.dtype == torch.float64 and shift.is_contiguous() and shift.numel() == nb):
+        raise ValueError("shift must be a contiguous float64 cuda tensor of nb values")
+    out = torch.empty((nb, n, n), dtype=torch.float32, device=B.device)
+    sing = torch.empty(nb, dtype=torch.int32, device=B.device)
+    lib = _ffi.load()
+    st = lib.dgx_block_inverse(nb, n, ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(shift.data_ptr()),
+                               ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(sing.data_ptr()),
+                               ctypes.c_void_p(torch.cuda.current_stream(B.device).cuda_stream))
+    _ffi.check(st)
+    bad = torch.nonzero(sing)
+    if bad.numel():
+        b = int(bad[0, 0])
+        raise FloatingPointError(f"block {b} is singular (no pivot in column {int(sing[b]) - 1})")
+    return out
 
 
 def solve_linear(solver, b, inv_dt=0.0, transpose=False, precond=None, rtol=1e-4, restart=60, max_iter=600):

@@ -128,6 +128,16 @@ dgx_status dgx_sync(dgx_ctx* ctx, int64_t* elem);
 dgx_status dgx_vjp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* v, double* out);
 dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* w, double* out);
 
+/* Batched block inversion for a block-Jacobi preconditioner: inv[b] = (shift[b] I - blocks[b])^-1 for
+ * n_blocks dense n x n blocks (n <= 320), blocks row-major FP64, inv row-major FP32, all device
+ * pointers; singular[b] = 0, or k + 1 when column k had no non-zero pivot (the block's output is then
+ * meaningless).  Gauss-Jordan with partial pivoting, one workgroup per block, launched on `stream`
+ * (a hipStream_t; NULL = the null stream) without synchronising.  Replaces the preconditioner set-up of
+ * the reference's KSP (constructor_petsc_ksp, solve_implicit.c:562-590: ILU on the assembled matrix),
+ * which a matrix-free solver cannot factor; the blocks come from dgx_jvp probes. */
+dgx_status dgx_block_inverse(int64_t n_blocks, int n, const double* blocks, const double* shift, float* inv,
+                             int32_t* singular, void* stream);
+
 /* Time of the last dgx_step / dgx_rk_stage kernel work in milliseconds, from HIP events recorded on
  * the context stream around the launches (0 when unavailable). */
 double dgx_last_elapsed_ms(const dgx_ctx* ctx);

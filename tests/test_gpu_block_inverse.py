@@ -1,0 +1,113 @@
+"""dgx_block_inverse (FP32 Gauss-Jordan with partial pivoting, one workgroup per block) on the GPU.
+
+- random blocks (shift I - B) of every tile shape from n = 1 to the limit 320, including the 32-column
+  tile edges (32, 33, 64, 65), against numpy's FP64 inverse: forward error within 2e-6 cond(A) (FP32
+  Gauss-Jordan with partial pivoting; the measured errors are far below), residual |A X - I| <= 1e-4;
+- a permuted identity and a permuted diagonal (every pivot off the diagonal, exact in FP32): the
+  inverse is reproduced exactly, which pins the pivot bookkeeping and the final unscrambling;
+- a singular block is reported (and its neighbours in the batch are unaffected); n > 320 and bad sizes
+  are refused; zero blocks is a no-op.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from dpgsolver_amd import _ffi, implicit
+
+pytestmark = pytest.mark.gpu
+
+
+def _require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _run(B, shift):
+    import torch
+    Bt = torch.from_numpy(np.ascontiguousarray(B, dtype=np.float64)).cuda()
+    st = torch.from_numpy(np.ascontiguousarray(shift, dtype=np.float64)).cuda()
+    return implicit.block_inverse(Bt, st).cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("n", [1, 5, 20, 32, 33, 64, 65, 100, 160, 250, 320])
+def test_random_blocks_against_numpy(n):
+    _require_gpu()
+    rng = np.random.default_rng(n)
+    nb = 37
+    B = rng.standard_normal((nb, n, n)) / np.sqrt(n)
+    shift = 1.5 + rng.random(nb)
+    print(f"block inverse n={n} nb={nb}", flush=True)
+    X = _run(B, shift)
+    for b in range(nb):
+        A = shift[b] * np.eye(n) - B[b]
+        ref = np.linalg.inv(A)
+        cond = np.linalg.cond(A)
+        err = np.linalg.norm(X[b] - ref) / np.linalg.norm(ref)
+        assert err <= 2e-6 * cond, (b, err, cond)
+        assert np.abs(A @ X[b] - np.eye(n)).max() <= 1e-4
+
+
+@pytest.mark.parametrize("n", [7, 33, 320])
+def test_permutation_blocks_exact(n):
+    """Zero diagonal everywhere: every pivot comes from another row; the inverse of P D is D^-1 P^T,
+    with powers of two on D so that FP32 represents everything exactly."""
+    _require_gpu()
+    rng = np.random.default_rng(100 + n)
+    nb = 5
+    B = np.zeros((nb, n, n))
+    refs = []
+    for b in range(nb):
+        perm = rng.permutation(n)
+        while n > 1 and np.any(perm == np.arange(n)):
+            perm = rng.permutation(n)
+        d = 2.0 ** rng.integers(-3, 4, size=n)
+        A = np.zeros((n, n))
+        A[np.arange(n), perm] = d
+        B[b] = -A                          # shift 0: A = -B
+        refs.append(np.linalg.inv(A))
+    X = _run(B, np.zeros(nb))
+    for b in range(nb):
+        assert np.array_equal(X[b], refs[b]), b
+
+
+def test_singular_block_reported_and_neighbours_intact():
+    _require_gpu()
+    import torch
+    rng = np.random.default_rng(7)
+    n, nb = 40, 4
+    B = rng.standard_normal((nb, n, n)) / np.sqrt(n)
+    shift = np.full(nb, 2.0)
+    B[2, 5, :] = 0.0
+    B[2, 5, 5] = 2.0                       # row 5 of shift I - B is zero
+    Bt = torch.from_numpy(B).cuda()
+    st = torch.from_numpy(shift).cuda()
+    out = torch.empty((nb, n, n), dtype=torch.float32, device="cuda")
+    sing = torch.full((nb,), -1, dtype=torch.int32, device="cuda")
+    lib = _ffi.load()
+    status = lib.dgx_block_inverse(nb, n, ctypes.c_void_p(Bt.data_ptr()), ctypes.c_void_p(st.data_ptr()),
+                                   ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(sing.data_ptr()), None)
+    assert status == _ffi.DGX_OK
+    torch.cuda.synchronize()
+    s = sing.cpu().numpy()
+    assert s[2] > 0 and s[0] == 0 and s[1] == 0 and s[3] == 0
+    X = out.cpu().numpy().astype(np.float64)
+    for b in (0, 1, 3):
+        ref = np.linalg.inv(shift[b] * np.eye(n) - B[b])
+        assert np.linalg.norm(X[b] - ref) / np.linalg.norm(ref) <= 1e-5
+    with pytest.raises(FloatingPointError):
+        implicit.block_inverse(Bt, st)
+
+
+def test_argument_checks():
+    _require_gpu()
+    import torch
+    lib = _ffi.load()
+    buf = torch.zeros(4, dtype=torch.float64, device="cuda")
+    p = ctypes.c_void_p(buf.data_ptr())
+    assert lib.dgx_block_inverse(1, 321, p, p, p, p, None) == _ffi.DGX_ERR_INVALID_ARG
+    assert lib.dgx_block_inverse(1, 0, p, p, p, p, None) == _ffi.DGX_ERR_INVALID_ARG
+    assert lib.dgx_block_inverse(-1, 4, p, p, p, p, None) == _ffi.DGX_ERR_INVALID_ARG
+    assert lib.dgx_block_inverse(1, 4, None, p, p, p, None) == _ffi.DGX_ERR_INVALID_ARG
+    assert lib.dgx_block_inverse(0, 4, None, None, None, None, None) == _ffi.DGX_OK
