@@ -278,6 +278,58 @@ def bench_implicit(args, dg, torch):
     print(json.dumps(out), flush=True)
 
 
+def bench_geometry(args, dg, torch):
+    """Geometry set-up (metric terms, det J, normals, face det J) of the n^3 curved HEX p=3 mesh (p_g = 3)
+    on the GPU, device outputs; one unit = one build of the whole mesh.  The CPU baseline is the host
+    build (dgx_geometry_build, OpenMP over volumes) on a bounded sample of the same construction."""
+    n, p = args.geo_n, 3
+    L = 2 * math.pi
+    mesh = dg.structured_mesh(3, (n, n, n), (0, 0, 0), (L, L, L), p_g=p, amp=0.05, seed=args.seed)
+    dg.geometry_gpu(dg.structured_mesh(3, (2, 2, 2), (0, 0, 0), (L, L, L), p_g=p, amp=0.05), p, device=True)
+    kms, walls = [], []
+    for _ in range(max(args.steps, 1)):
+        tm = {}
+        t0 = time.perf_counter()
+        g = dg.geometry_gpu(mesh, p, device=True, timing=tm)
+        walls.append(time.perf_counter() - t0)
+        kms.append(tm["kernel_ms"])
+        del g
+    k_ms = sorted(kms)[len(kms) // 2]
+    wall = sorted(walls)[len(walls) // 2]
+    E, F = mesh.n_elem, mesh.n_face
+    Ng, Nc, Nf = (p + 1) ** 3, (p + 1) ** 3, (p + 1) ** 2
+    bytes_alg = 8 * (E * 3 * Ng + E * 9 * Nc + E * Nc + F * Nf * 4) + 8 * F + F * 2
+    achieved = bytes_alg / (k_ms * 1e-3) / 1e9
+    out = {"metric": "geometry set-up of a curved HEX p=3 mesh on the GPU", "value": E / (wall), "unit": "volumes/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"geometry of the {n}^3 curved HEX mesh (p_g = 3, 0.05h sinusoidal perturbation) "
+                                  f"for p = 3: metric terms (3-D curl form), det J, normals, face det J",
+                      "elements": E, "faces": F, "kernel_ms": k_ms,
+                      "wall_includes": "host face grouping, upload of the geometry nodes, kernel"},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "dgx::geometry_kernel<3>",
+                        "algorithmic_bytes_per_launch": bytes_alg}}
+    if not args.no_cpu_baseline:
+        ns = args.cpu_sample_n
+        sm = dg.structured_mesh(3, (ns, ns, ns), (0, 0, 0), (L, L, L), p_g=p, amp=0.05, seed=args.seed)
+        builds = 0
+        t0 = time.perf_counter()
+        while True:
+            dg.geometry(sm, p)
+            builds += 1
+            el = time.perf_counter() - t0
+            if el > args.cpu_seconds:
+                break
+        omp = os.environ.get("OMP_NUM_THREADS", "")
+        cores = int(omp) if omp.isdigit() and int(omp) > 0 else os.cpu_count()
+        out["cpu_baseline"] = {"value": sm.n_elem * builds / el, "unit": "volumes/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"host dgx_geometry_build (C, OpenMP, {cores} threads) on the {ns}^3 mesh "
+                                         f"of the same construction, {builds} builds in {el:.1f} s ({cpu_model()})"}
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline_jvp(args, dg, mesh, geo, u0, p, timed):
     """The oracle's complex-step JVP of M^-1 R (the CPU restatement's linearization) on the same mesh
     and state, on all host threads, for about --cpu-seconds; unit JVP/s."""
@@ -306,7 +358,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="hex64", choices=["hex64", "hex32", "vortex2d", "ns48", "vjp32", "nk"])
+    ap.add_argument("--config", default="hex64",
+                    choices=["hex64", "hex32", "vortex2d", "ns48", "vjp32", "nk", "geo"])
+    ap.add_argument("--geo-n", type=int, default=64)
     ap.add_argument("--nk-n", type=int, default=16)
     ap.add_argument("--newton-steps", type=int, default=4)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
@@ -332,6 +386,8 @@ def main():
 
     if args.config == "nk":
         return bench_implicit(args, dg, torch)
+    if args.config == "geo":
+        return bench_geometry(args, dg, torch)
 
     t_setup = time.perf_counter()
     viscous = None

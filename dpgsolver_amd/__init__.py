@@ -6,12 +6,12 @@ the host-side set-up helpers, the solver mirror and the matrix-free implicit sol
 linearized-residual kernels.
 """
 from . import _ffi
-from .problem import (Geometry, Mesh, dt_cfl, geometry, ic_isentropic_vortex, ic_taylor_green, ic_uniform_flow,
-                      randn, read_gmsh, solution_nodes_xyz, structured_mesh, tgv_3d, vortex_2d)
+from .problem import (Geometry, Mesh, dt_cfl, geometry, geometry_gpu, ic_isentropic_vortex, ic_taylor_green,
+                      ic_uniform_flow, randn, read_gmsh, solution_nodes_xyz, structured_mesh, tgv_3d, vortex_2d)
 from .solver import Solver, residual_function
 from .implicit import BlockJacobi, gmres, solve_implicit, solve_linear
 
 __all__ = ["Mesh", "Geometry", "Solver", "structured_mesh", "geometry", "ic_isentropic_vortex",
            "ic_taylor_green", "dt_cfl", "randn", "solution_nodes_xyz", "vortex_2d", "tgv_3d",
-           "residual_function", "read_gmsh", "ic_uniform_flow", "solve_implicit", "solve_linear", "gmres",
-           "BlockJacobi"]
+           "residual_function", "read_gmsh", "ic_uniform_flow", "geometry_gpu", "solve_implicit", "solve_linear",
+           "gmres", "BlockJacobi"]

@@ -102,6 +102,8 @@ SIGNATURES = {
                                           c_int32_p, c_int8_p, c_int8_p, c_int32_p, c_int32_p, c_int32_p]),
     "dgx_geometry_build": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, c_double_p, c_double_p,
                                           c_double_p, c_double_p]),
+    "dgx_geometry_build_gpu": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_double_p]),
     "dgx_solution_nodes_xyz": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, c_double_p]),
     "dgx_ic_isentropic_vortex": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, c_double_p]),

@@ -1,0 +1,383 @@
+// Geometry set-up on the GPU: the metric terms, det J, face normals and face det J that
+// dgx_geometry_build computes on the host (geometry_T.c:208-325,734-804; 3-D cofactors in the
+// conservative curl form of deprecated/src/setup_geom_factors.c:182-250), one workgroup per volume.
+//
+// The volume's geometry nodes (element-local coordinates, as on the host) sit in LDS; each thread owns
+// one geometry node for the derivative and curl phases and one cubature node for the interpolation
+// phase.  Faces are computed by their left volume from its metric terms (the host's grouping, built
+// here as a CSR list on the host: integer bookkeeping over the face list).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "dgx.h"
+#include "dgx_setup.h"
+
+namespace dgx {
+namespace {
+
+constexpr int NMAX = 5;                 // geometry and solution orders up to 4
+constexpr int VMAX = NMAX * NMAX * NMAX;
+constexpr int BLOCK = 128;
+
+struct GeoOps {
+    int ng, nc, Ng, Nc, Nf;
+    double Dg[NMAX * NMAX];             // d l_m / dx at the GLL geometry nodes, [a][m]
+    double Igc[NMAX * NMAX];            // l_m at the GL cubature nodes, [n][m]
+    double Dgc[NMAX * NMAX];            // d l_m / dx at the GL cubature nodes, [n][m]
+    double Igm[NMAX], Igp[NMAX];        // l_m at -1 / +1
+};
+
+__device__ inline void digits(int a, int n, int* out)
+{
+    out[0] = a % n;
+    out[1] = (a / n) % n;
+    out[2] = a / (n * n);
+}
+
+// d_j f at geometry node a (f on the ng^D grid in LDS)
+template <int D>
+__device__ inline double grad_g(const GeoOps& g, const double* f, int j, int a)
+{
+    int ia[3];
+    digits(a, g.ng, ia);
+    int stride = 1;
+    for (int k = 0; k < j; ++k)
+        stride *= g.ng;
+    const int base = a - ia[j] * stride;
+    double s = 0.0;
+    for (int m = 0; m < g.ng; ++m)
+        s = fma(g.Dg[ia[j] * NMAX + m], f[base + m * stride], s);
+    return s;
+}
+
+// Wc[(w * Ng + a) * Nc + n]: weight of geometry node a at cubature node n, w = 0 interpolation,
+// w = 1 + j the derivative along j; Wf[(lf * Ng + a) * Nf + q]: interpolation to the face nodes of lf
+// (tangential directions in order).  Products of the 1-D operators, built once on the host; a-major, so
+// that consecutive threads (nodes) read consecutive weights.
+template <int D>
+__global__ __launch_bounds__(BLOCK) void geometry_kernel(GeoOps gk, const double* __restrict__ xyz_g, int64_t E,
+                                                         const double* __restrict__ Wc,
+                                                         const double* __restrict__ Wf,
+                                                         const int64_t* __restrict__ ffirst,
+                                                         const int64_t* __restrict__ flist,
+                                                         const int8_t* __restrict__ face_lf,
+                                                         double* __restrict__ met, double* __restrict__ jdet,
+                                                         double* __restrict__ nrm, double* __restrict__ jdf,
+                                                         int* __restrict__ bad)
+{
+    __shared__ double sX[3][VMAX];
+    __shared__ double sJ[9][VMAX];
+    __shared__ double sV[3][VMAX];
+    __shared__ double sM[9][VMAX];
+    __shared__ GeoOps g;                // the 1-D operators, read with per-thread indices
+    const int64_t e = blockIdx.x;
+    const int t = threadIdx.x;
+    if (t == 0)
+        g = gk;
+    __syncthreads();
+    const int Ng = gk.Ng, Nc = gk.Nc, Nf = gk.Nf;
+    const double* Xe = xyz_g + e * D * Ng;
+
+    // element-local coordinates: the curl form's products X_a dX_b at the element's own scale
+    for (int q = t; q < Ng; q += BLOCK)
+        for (int i = 0; i < D; ++i)
+            sX[i][q] = Xe[i * Ng + q] - Xe[i * Ng];
+    __syncthreads();
+    for (int q = t; q < Ng; q += BLOCK)
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j)
+                sJ[i * 3 + j][q] = grad_g<D>(g, sX[i], j, q);
+    __syncthreads();
+
+    if constexpr (D == 2) {
+        for (int q = t; q < Ng; q += BLOCK) {
+            sM[0 + 2 * 0][q] = sJ[1 * 3 + 1][q];        //  y_s
+            sM[0 + 2 * 1][q] = -sJ[1 * 3 + 0][q];       // -y_r
+            sM[1 + 2 * 0][q] = -sJ[0 * 3 + 1][q];       // -x_s
+            sM[1 + 2 * 1][q] = sJ[0 * 3 + 0][q];        //  x_r
+        }
+    } else {
+        // C_ic = 1/2 [curl(X_a grad X_b) - curl(X_b grad X_a)]_c, (a, b) = (i+1, i+2) mod 3
+        for (int q = t; q < Ng; q += BLOCK)
+            for (int c = 0; c < 9; ++c)
+                sM[c][q] = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            const int ia = (i + 1) % 3, ib = (i + 2) % 3;
+            for (int s = 0; s < 2; ++s) {
+                const int a = s ? ib : ia, b = s ? ia : ib;
+                __syncthreads();                        // sV free (previous curl done)
+                for (int q = t; q < Ng; q += BLOCK)
+                    for (int k = 0; k < 3; ++k)
+                        sV[k][q] = sX[a][q] * sJ[b * 3 + k][q];
+                __syncthreads();
+                const double sg = s ? -0.5 : 0.5;
+                for (int q = t; q < Ng; q += BLOCK)
+                    for (int c = 0; c < 3; ++c) {
+                        const int k1 = (c + 1) % 3, k2 = (c + 2) % 3;
+                        const double curl = grad_g<D>(g, sV[k2], k1, q) - grad_g<D>(g, sV[k1], k2, q);
+                        sM[i + 3 * c][q] += sg * curl;
+                    }
+            }
+        }
+    }
+    __syncthreads();
+
+    for (int n = t; n < Nc; n += BLOCK) {
+        double m[D * D], J[D][D];
+#pragma unroll
+        for (int c = 0; c < D * D; ++c)
+            m[c] = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                J[i][j] = 0.0;
+        for (int a = 0; a < Ng; ++a) {
+            const double w0 = Wc[(int64_t)a * Nc + n];
+#pragma unroll
+            for (int c = 0; c < D * D; ++c)
+                m[c] = fma(w0, sM[c][a], m[c]);
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const double wj = Wc[((int64_t)(1 + j) * Ng + a) * Nc + n];
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+                    J[i][j] = fma(wj, sX[i][a], J[i][j]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < D * D; ++c)
+            met[(e * D * D + c) * Nc + n] = m[c];
+        double det;
+        if constexpr (D == 2)
+            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        else
+            det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                  J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        jdet[e * Nc + n] = det;
+        if (!(det > 0.0))
+            atomicOr(bad, 1);
+    }
+
+    // faces whose left volume is e: thread t serves face ffirst[e] + t / Nf, node t % Nf
+    const int64_t f0 = ffirst[e], nfe = ffirst[e + 1] - f0;
+    for (int64_t u = t; u < nfe * Nf; u += BLOCK) {
+        const int64_t f = flist[f0 + u / Nf];
+        const int q = (int)(u % Nf);
+        const int lf = face_lf[2 * f];
+        const int dn = lf / 2;
+        const double sgn = (lf & 1) ? 1.0 : -1.0;
+        double v[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+            v[i] = 0.0;
+        for (int a = 0; a < Ng; ++a) {
+            const double w = Wf[((int64_t)lf * Ng + a) * Nf + q];
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+                v[i] = fma(w, sM[i + D * dn][a], v[i]);
+        }
+        double s2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            v[i] *= sgn;
+            s2 += v[i] * v[i];
+        }
+        const double sn = sqrt(s2);
+        jdf[f * Nf + q] = sn;
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+            nrm[(f * Nf + q) * D + i] = v[i] / sn;
+    }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+}  // namespace dgx
+
+extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int ptr_kind, double* metrics_vc,
+                                             double* jdet_vc, double* normals_fc, double* jdet_fc, double* kernel_ms)
+{
+    using namespace dgx;
+    if (!mesh || !mesh->xyz_g || !mesh->face_elem || !mesh->face_lf || !metrics_vc || !jdet_vc ||
+        !normals_fc || !jdet_fc || (mesh->d != 2 && mesh->d != 3) || p < 1 || p > NMAX - 1 ||
+        mesh->p_g < 1 || mesh->p_g > NMAX - 1 || mesh->n_elem < 1 || mesh->n_elem > 0x7fffffff ||
+        mesh->n_face < 0 || (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
+        return DGX_ERR_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return DGX_ERR_NO_DEVICE;
+    const int d = mesh->d;
+    const int64_t E = mesh->n_elem, F = mesh->n_face;
+
+    GeoOps g;
+    std::memset(&g, 0, sizeof(g));
+    g.ng = mesh->p_g + 1;
+    g.nc = p + 1;
+    g.Ng = g.ng * g.ng * (d == 3 ? g.ng : 1);
+    g.Nc = g.nc * g.nc * (d == 3 ? g.nc : 1);
+    g.Nf = d == 3 ? g.nc * g.nc : g.nc;
+    {
+        double xg[8], wg[8], xc[8], wc[8], tmp[64];
+        const double m1 = -1.0, p1 = 1.0;
+        if (dgx_nodes_1d(DGX_NODES_GLL, mesh->p_g, xg, wg) != DGX_OK ||
+            dgx_nodes_1d(DGX_NODES_GL, p, xc, wc) != DGX_OK)
+            return DGX_ERR_INVALID_ARG;
+        const int ng = g.ng, nc = g.nc;
+        dgx_lagrange_1d(ng, xg, ng, xg, 1, tmp);
+        for (int a = 0; a < ng; ++a)
+            for (int m = 0; m < ng; ++m)
+                g.Dg[a * NMAX + m] = tmp[a * ng + m];
+        dgx_lagrange_1d(ng, xg, nc, xc, 0, tmp);
+        for (int n = 0; n < nc; ++n)
+            for (int m = 0; m < ng; ++m)
+                g.Igc[n * NMAX + m] = tmp[n * ng + m];
+        dgx_lagrange_1d(ng, xg, nc, xc, 1, tmp);
+        for (int n = 0; n < nc; ++n)
+            for (int m = 0; m < ng; ++m)
+                g.Dgc[n * NMAX + m] = tmp[n * ng + m];
+        dgx_lagrange_1d(ng, xg, 1, &m1, 0, g.Igm);
+        dgx_lagrange_1d(ng, xg, 1, &p1, 0, g.Igp);
+    }
+
+    // weight tables (see geometry_kernel)
+    const int Ng = g.Ng, Nc = g.Nc, Nf = g.Nf, ng = g.ng, nc = g.nc;
+    std::vector<double> wc((size_t)(d + 1) * Ng * Nc), wf((size_t)2 * d * Ng * Nf);
+    for (int w = 0; w <= d; ++w)
+        for (int a = 0; a < Ng; ++a)
+            for (int n = 0; n < Nc; ++n) {
+                const int ia[3] = { a % ng, (a / ng) % ng, a / (ng * ng) };
+                const int in[3] = { n % nc, (n / nc) % nc, n / (nc * nc) };
+                double v = 1.0;
+                for (int k = 0; k < d; ++k)
+                    v *= (w == 1 + k ? g.Dgc : g.Igc)[in[k] * NMAX + ia[k]];
+                wc[((size_t)w * Ng + a) * Nc + n] = v;
+            }
+    for (int lf = 0; lf < 2 * d; ++lf)
+        for (int a = 0; a < Ng; ++a)
+            for (int q = 0; q < Nf; ++q) {
+                const int ia[3] = { a % ng, (a / ng) % ng, a / (ng * ng) };
+                int iq[3] = { 0, 0, 0 }, r = q;
+                for (int k = 0; k < d; ++k)
+                    if (k != lf / 2) {
+                        iq[k] = r % nc;
+                        r /= nc;
+                    }
+                double v = 1.0;
+                for (int k = 0; k < d; ++k)
+                    v *= (k == lf / 2) ? ((lf & 1) ? g.Igp : g.Igm)[ia[k]] : g.Igc[iq[k] * NMAX + ia[k]];
+                wf[((size_t)lf * Ng + a) * Nf + q] = v;
+            }
+
+    // faces grouped by their left volume (CSR)
+    std::vector<int64_t> first((size_t)E + 1, 0), list((size_t)(F > 0 ? F : 1));
+    for (int64_t f = 0; f < F; ++f) {
+        const int64_t eL = mesh->face_elem[2 * f];
+        const int lf = mesh->face_lf[2 * f];
+        if (eL < 0 || eL >= E || lf < 0 || lf >= 2 * d)
+            return DGX_ERR_INVALID_ARG;
+        first[(size_t)eL + 1]++;
+    }
+    for (int64_t e = 0; e < E; ++e)
+        first[(size_t)e + 1] += first[(size_t)e];
+    {
+        std::vector<int64_t> fill(first.begin(), first.end() - 1);
+        for (int64_t f = 0; f < F; ++f)
+            list[(size_t)fill[(size_t)mesh->face_elem[2 * f]]++] = f;
+    }
+    std::vector<int8_t> lfs((size_t)(2 * (F > 0 ? F : 1)), 0);
+    for (int64_t f = 0; f < 2 * F; ++f)
+        lfs[(size_t)f] = mesh->face_lf[f];
+
+    const size_t b_xyz = sizeof(double) * (size_t)E * d * g.Ng;
+    const size_t b_met = sizeof(double) * (size_t)E * d * d * g.Nc, b_jd = sizeof(double) * (size_t)E * g.Nc;
+    const size_t b_nrm = sizeof(double) * (size_t)(F > 0 ? F : 1) * g.Nf * d;
+    const size_t b_jdf = sizeof(double) * (size_t)(F > 0 ? F : 1) * g.Nf;
+    DevBuf dx, dwc, dwf, dfirst, dlist, dlf, dbad, dmet, djd, dnrm, djdf;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    dgx_status rc = DGX_OK;
+    int bad = 0;
+#define GEO_TRY(x)                        \
+    do {                                  \
+        if ((x) != hipSuccess) {          \
+            rc = DGX_ERR_HIP;             \
+            goto done;                    \
+        }                                 \
+    } while (0)
+    GEO_TRY(hipStreamCreate(&st));
+    GEO_TRY(hipEventCreate(&ev0));
+    GEO_TRY(hipEventCreate(&ev1));
+    GEO_TRY(hipMalloc(&dx.p, b_xyz));
+    GEO_TRY(hipMalloc(&dwc.p, sizeof(double) * wc.size()));
+    GEO_TRY(hipMalloc(&dwf.p, sizeof(double) * wf.size()));
+    GEO_TRY(hipMalloc(&dfirst.p, sizeof(int64_t) * first.size()));
+    GEO_TRY(hipMalloc(&dlist.p, sizeof(int64_t) * list.size()));
+    GEO_TRY(hipMalloc(&dlf.p, lfs.size()));
+    GEO_TRY(hipMalloc(&dbad.p, sizeof(int)));
+    if (ptr_kind == DGX_PTR_HOST) {
+        GEO_TRY(hipMalloc(&dmet.p, b_met));
+        GEO_TRY(hipMalloc(&djd.p, b_jd));
+        GEO_TRY(hipMalloc(&dnrm.p, b_nrm));
+        GEO_TRY(hipMalloc(&djdf.p, b_jdf));
+    }
+    {
+        double* om = ptr_kind == DGX_PTR_HOST ? (double*)dmet.p : metrics_vc;
+        double* oj = ptr_kind == DGX_PTR_HOST ? (double*)djd.p : jdet_vc;
+        double* on = ptr_kind == DGX_PTR_HOST ? (double*)dnrm.p : normals_fc;
+        double* of = ptr_kind == DGX_PTR_HOST ? (double*)djdf.p : jdet_fc;
+        GEO_TRY(hipMemcpyAsync(dx.p, mesh->xyz_g, b_xyz, hipMemcpyHostToDevice, st));
+        GEO_TRY(hipMemcpyAsync(dwc.p, wc.data(), sizeof(double) * wc.size(), hipMemcpyHostToDevice, st));
+        GEO_TRY(hipMemcpyAsync(dwf.p, wf.data(), sizeof(double) * wf.size(), hipMemcpyHostToDevice, st));
+        GEO_TRY(hipMemcpyAsync(dfirst.p, first.data(), sizeof(int64_t) * first.size(), hipMemcpyHostToDevice, st));
+        GEO_TRY(hipMemcpyAsync(dlist.p, list.data(), sizeof(int64_t) * list.size(), hipMemcpyHostToDevice, st));
+        GEO_TRY(hipMemcpyAsync(dlf.p, lfs.data(), lfs.size(), hipMemcpyHostToDevice, st));
+        GEO_TRY(hipMemcpyAsync(dbad.p, &bad, sizeof(int), hipMemcpyHostToDevice, st));
+        GEO_TRY(hipEventRecord(ev0, st));
+        if (d == 2)
+            hipLaunchKernelGGL(geometry_kernel<2>, dim3((unsigned)E), dim3(BLOCK), 0, st, g, (const double*)dx.p, E,
+                               (const double*)dwc.p, (const double*)dwf.p, (const int64_t*)dfirst.p,
+                               (const int64_t*)dlist.p, (const int8_t*)dlf.p, om, oj, on, of, (int*)dbad.p);
+        else
+            hipLaunchKernelGGL(geometry_kernel<3>, dim3((unsigned)E), dim3(BLOCK), 0, st, g, (const double*)dx.p, E,
+                               (const double*)dwc.p, (const double*)dwf.p, (const int64_t*)dfirst.p,
+                               (const int64_t*)dlist.p, (const int8_t*)dlf.p, om, oj, on, of, (int*)dbad.p);
+        GEO_TRY(hipGetLastError());
+        GEO_TRY(hipEventRecord(ev1, st));
+        if (ptr_kind == DGX_PTR_HOST) {
+            GEO_TRY(hipMemcpyAsync(metrics_vc, om, b_met, hipMemcpyDeviceToHost, st));
+            GEO_TRY(hipMemcpyAsync(jdet_vc, oj, b_jd, hipMemcpyDeviceToHost, st));
+            if (F > 0) {
+                GEO_TRY(hipMemcpyAsync(normals_fc, on, b_nrm, hipMemcpyDeviceToHost, st));
+                GEO_TRY(hipMemcpyAsync(jdet_fc, of, b_jdf, hipMemcpyDeviceToHost, st));
+            }
+        }
+        GEO_TRY(hipMemcpyAsync(&bad, dbad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        GEO_TRY(hipStreamSynchronize(st));
+        if (kernel_ms) {
+            float ms = 0.f;
+            GEO_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+            *kernel_ms = ms;
+        }
+    }
+    if (bad)
+        rc = DGX_ERR_GEOMETRY;
+done:
+#undef GEO_TRY
+    if (st)
+        (void)hipStreamSynchronize(st), (void)hipStreamDestroy(st);
+    if (ev0)
+        (void)hipEventDestroy(ev0);
+    if (ev1)
+        (void)hipEventDestroy(ev1);
+    return rc;
+}

@@ -134,6 +134,30 @@ def geometry(mesh, p):
     return g
 
 
+def geometry_gpu(mesh, p, device=False, timing=None):
+    """geometry(mesh, p) computed on the GPU (dgx_geometry_build_gpu; p, p_g <= 4).  device=False: numpy
+    arrays (a Geometry); device=True: a Geometry of float64 torch cuda tensors, never copied to the host.
+    timing: an optional dict that receives the kernel time ("kernel_ms")."""
+    lib = _ffi.load()
+    d = mesh.d
+    Nc, Nf = (p + 1) ** d, (p + 1) ** (d - 1)
+    shapes = ((mesh.n_elem, d, d, Nc), (mesh.n_elem, Nc), (mesh.n_face, Nf, d), (mesh.n_face, Nf))
+    if device:
+        import torch
+        arrs = [torch.empty(sh, dtype=torch.float64, device="cuda") for sh in shapes]
+        ptrs = [ctypes.c_void_p(a.data_ptr()) for a in arrs]
+        kind = _ffi.PTR_DEVICE
+    else:
+        arrs = [np.empty(sh) for sh in shapes]
+        ptrs = [ctypes.c_void_p(a.ctypes.data) for a in arrs]
+        kind = _ffi.PTR_HOST
+    kms = ctypes.c_double(0.0)
+    _ffi.check(lib.dgx_geometry_build_gpu(mesh.c_struct(), p, kind, *ptrs, ctypes.byref(kms)))
+    if timing is not None:
+        timing["kernel_ms"] = kms.value
+    return Geometry(*arrs)
+
+
 def solution_nodes_xyz(mesh, p):
     lib = _ffi.load()
     out = np.empty((mesh.n_elem, mesh.d, (p + 1) ** mesh.d))

@@ -100,6 +100,16 @@ dgx_status dgx_mesh_read_gmsh(const char* path, int d, int64_t* n_elem, int64_t*
 dgx_status dgx_geometry_build(const dgx_mesh* mesh, int p,
                               double* metrics_vc, double* jdet_vc, double* normals_fc, double* jdet_fc);
 
+/* dgx_geometry_build on the GPU (one workgroup per volume; same definitions, outputs and layouts),
+ * for p <= 4 and p_g <= 4.  The mesh arrays are host pointers; the outputs are host or device pointers
+ * (ptr_kind DGX_PTR_HOST / DGX_PTR_DEVICE).  Synchronous; kernel_ms (nullable) receives the kernel's
+ * time from HIP events around its launch.  DGX_ERR_GEOMETRY when det J <= 0 at any
+ * cubature node, DGX_ERR_NO_DEVICE without a GPU.  Replaces the geometry set-up of the reference
+ * (setup_geometry, geometry_T.c:208-325,734-804) for large meshes, where the host build dominates the
+ * set-up time. */
+dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int ptr_kind, double* metrics_vc,
+                                  double* jdet_vc, double* normals_fc, double* jdet_fc, double* kernel_ms);
+
 /* Physical coordinates of the solution nodes: xyz_s[n_elem][d][Ns] (interpolated from the geometry nodes). */
 dgx_status dgx_solution_nodes_xyz(const dgx_mesh* mesh, int p, double* xyz_s);
 
