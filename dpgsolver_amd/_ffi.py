@@ -104,6 +104,9 @@ SIGNATURES = {
                                           c_double_p, c_double_p]),
     "dgx_geometry_build_gpu": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_double_p]),
+    "dgx_restart_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(DgxMesh), ctypes.c_int, c_double_p]),
+    "dgx_restart_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, c_double_p]),
+    "dgx_vtu_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(DgxMesh), ctypes.c_int, c_double_p]),
     "dgx_solution_nodes_xyz": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, c_double_p]),
     "dgx_ic_isentropic_vortex": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, c_double_p]),

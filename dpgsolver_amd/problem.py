@@ -158,6 +158,28 @@ def geometry_gpu(mesh, p, device=False, timing=None):
     return Geometry(*arrs)
 
 
+def write_restart(path, mesh, p, u):
+    """Restart file of the state u [E][d+2][Ns] in the reference's format (dgx_restart_write)."""
+    lib = _ffi.load()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    _ffi.check(lib.dgx_restart_write(str(path).encode(), mesh.c_struct(), p, _ffi.dptr(u)))
+
+
+def read_restart(path, mesh, p):
+    """The state [E][d+2][Ns] of a restart file written for this mesh and order (dgx_restart_read)."""
+    lib = _ffi.load()
+    u = np.empty((mesh.n_elem, mesh.d + 2, (p + 1) ** mesh.d))
+    _ffi.check(lib.dgx_restart_read(str(path).encode(), mesh.d, p, mesh.n_elem, _ffi.dptr(u)))
+    return u
+
+
+def write_vtu(path, mesh, p, u):
+    """VTK XML solution file of the Euler state u (dgx_vtu_write)."""
+    lib = _ffi.load()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    _ffi.check(lib.dgx_vtu_write(str(path).encode(), mesh.c_struct(), p, _ffi.dptr(u)))
+
+
 def solution_nodes_xyz(mesh, p):
     lib = _ffi.load()
     out = np.empty((mesh.n_elem, mesh.d, (p + 1) ** mesh.d))

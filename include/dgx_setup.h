@@ -110,6 +110,26 @@ dgx_status dgx_geometry_build(const dgx_mesh* mesh, int p,
 dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int ptr_kind, double* metrics_vc,
                                   double* jdet_vc, double* normals_fc, double* jdet_fc, double* kernel_ms);
 
+/* Restart file of the state sol[n_elem][d+2][Ns] (reference layout, host) in the reference's format
+ * (output_restart_gmsh, test_case/solution/restart/restart_writers.c:149-420): a Gmsh 2.2 mesh whose
+ * nodes are the volumes' vertices sorted (last coordinate first) and made unique within a relative
+ * 1e-10, its volumes (type 3 QUAD / 5 HEX, 2 tags), and $SolutionCoefficients: per volume
+ * "index p_ref n_dof n_var" and the Bezier coefficients of the solution, (n_dof, n_var) column-major,
+ * "% .15e".  Periodic meshes have no boundary faces, so no face elements are written. */
+dgx_status dgx_restart_write(const char* path, const dgx_mesh* mesh, int p, const double* sol);
+
+/* The state of a restart file written for the same mesh (same volume order and order p): Bezier
+ * coefficients back to nodal values.  The reference's reader also handles non-nested meshes by
+ * locating nodes in the restart mesh (restart_T.c); this one requires the same mesh and returns
+ * DGX_ERR_INVALID_ARG for another volume count, order or variable count. */
+dgx_status dgx_restart_read(const char* path, int d, int p, int64_t n_elem, double* sol);
+
+/* VTK XML unstructured-grid file of the Euler solution (output_visualization_vtk_sol,
+ * visualization.c:302-349,675-865): one Piece per volume on the equally spaced plotting nodes of
+ * order p (nodes.c:113-120) split into p^d linear cells (nodes_plotting.c:255-276), with the point data
+ * index, p_ref, ml, density, velocity, pressure, entropy log(p rho^-gamma) and Mach number. */
+dgx_status dgx_vtu_write(const char* path, const dgx_mesh* mesh, int p, const double* sol);
+
 /* Physical coordinates of the solution nodes: xyz_s[n_elem][d][Ns] (interpolated from the geometry nodes). */
 dgx_status dgx_solution_nodes_xyz(const dgx_mesh* mesh, int p, double* xyz_s);
 

@@ -1,0 +1,177 @@
+"""Restart and visualisation output (dgx_restart_write / dgx_restart_read / dgx_vtu_write), on the CPU.
+
+- restart file of a 2 x 2 periodic QUAD mesh against the file the reference's rules give by hand:
+  vertices sorted by y then x (cmp_DIM_d compares the last coordinate first) and made unique, volumes as
+  "id 3 2 0 0" + vertices in Gmsh order, no face elements (periodic), then $SolutionCoefficients;
+- the Bezier coefficients in the file: a density linear in x on straight volumes has the coefficients
+  rho(x_i) at equally spaced x_i (linear precision of the Bernstein basis), in (n_dof, n_var) column-major;
+- the reference's own periodic mesh: the restart nodes are exactly its vertex set (at "%g") and the file
+  reads back as a Gmsh 2.2 mesh with the same volumes;
+- write -> read returns the state to 1e-14 (16 significant digits, Bezier round trip); a file for
+  another order or volume count is refused;
+- the VTU file parses as XML with one Piece per volume; points are the equally spaced plotting nodes;
+  a state polynomial of degree <= p is reproduced exactly at them (density, velocity, pressure) and the
+  entropy log(p rho^-gamma) and Mach number follow from them (2-D and 3-D).
+"""
+import math
+import os
+import xml.etree.ElementTree as ET
+
+import numpy as np
+import pytest
+
+import dpgsolver_amd as dg
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GAMMA = 1.4
+
+
+def _state_from(mesh, p, rho_f, vel_f, p_f):
+    x = dg.solution_nodes_xyz(mesh, p)
+    X = [x[:, k] for k in range(mesh.d)]
+    rho = rho_f(*X)
+    vel = [f(*X) for f in vel_f]
+    pr = p_f(*X)
+    ke = 0.5 * rho * sum(v * v for v in vel)
+    return np.stack([rho] + [rho * v for v in vel] + [pr / (GAMMA - 1.0) + ke], axis=1)
+
+
+def _sections(path):
+    lines = open(path).read().splitlines()
+    out, cur = {}, None
+    for ln in lines:
+        if ln.startswith("$End"):
+            cur = None
+        elif ln.startswith("$"):
+            cur = ln[1:]
+            out[cur] = []
+        elif cur is not None:
+            out[cur].append(ln)
+    return out
+
+
+def test_restart_known_answer_2x2(tmp_path):
+    mesh = dg.structured_mesh(2, (2, 2), (0, 0), (1, 1))
+    u = dg.ic_isentropic_vortex(mesh, 1)
+    path = tmp_path / "r.msh"
+    dg.write_restart(path, mesh, 1, u)
+    sec = _sections(path)
+    assert sec["MeshFormat"] == ["2.2 0 8"]
+    assert sec["Nodes"] == ["9", "1 0 0 0", "2 0.5 0 0", "3 1 0 0", "4 0 0.5 0", "5 0.5 0.5 0", "6 1 0.5 0",
+                            "7 0 1 0", "8 0.5 1 0", "9 1 1 0"]
+    # volumes in the structured order (x fastest); vertices counter-clockwise from the lower left
+    assert sec["Elements"] == ["4", "1 3 2 0 0 1 2 5 4", "2 3 2 0 0 2 3 6 5", "3 3 2 0 0 4 5 8 7",
+                               "4 3 2 0 0 5 6 9 8"]
+    assert sec["SolutionCoefficients"][0] == "4"
+    assert len(sec["SolutionCoefficients"]) == 5
+    assert sec["SolutionCoefficients"][1].startswith("1 1 4 4 ")
+    assert list(sec) == ["MeshFormat", "Nodes", "Elements", "SolutionCoefficients"]
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_restart_bezier_coefficients_linear_precision(tmp_path, p):
+    mesh = dg.structured_mesh(2, (3, 2), (0, 0), (3, 1))
+    u = _state_from(mesh, p, lambda x, y: 1.0 + 0.5 * x, (lambda x, y: 0.2 + 0 * x, lambda x, y: 0 * x),
+                    lambda x, y: 1.0 + 0 * x)
+    path = tmp_path / "r.msh"
+    dg.write_restart(path, mesh, p, u)
+    rows = _sections(path)["SolutionCoefficients"][1:]
+    n1 = p + 1
+    for e, row in enumerate(rows):
+        vals = row.split()
+        assert [int(v) for v in vals[:4]] == [e + 1, p, n1 * n1, 4]
+        coef = np.array([float(v) for v in vals[4:]]).reshape(4, n1 * n1)    # (n_dof, n_var) column-major
+        x0 = mesh.xyz_g[e, 0].min()
+        for j in range(n1):
+            for i in range(n1):
+                assert abs(coef[0, i + n1 * j] - (1.0 + 0.5 * (x0 + i / p))) <= 1e-14
+        assert np.allclose(coef[1], 0.2 * coef[0], rtol=1e-14)      # rho u is linear too
+
+
+def test_restart_on_reference_mesh(tmp_path):
+    src = os.path.join(HERE, "golden", "meshes", "straight_2d_quad_periodic.msh")
+    mesh, conn = dg.read_gmsh(src, 2)
+    u = dg.ic_isentropic_vortex(mesh, 2)
+    path = tmp_path / "r.msh"
+    dg.write_restart(path, mesh, 2, u)
+    nodes = np.array([[float(v) for v in ln.split()[1:3]] for ln in _sections(path)["Nodes"][1:]])
+    verts = np.unique(mesh.xyz_g.transpose(0, 2, 1).reshape(-1, 2), axis=0)
+    assert len(nodes) == len(verts)
+    for v in verts:
+        assert np.min(np.abs(nodes - v).max(axis=1)) <= 1e-5 * max(1.0, np.abs(v).max())
+    m2, _ = dg.read_gmsh(path, 2)                 # a Gmsh 2.2 file our reader accepts
+    assert m2.n_elem == mesh.n_elem
+    assert np.allclose(m2.xyz_g, mesh.xyz_g, atol=1e-5)
+    v = dg.read_restart(path, mesh, 2)
+    assert np.abs(v - u).max() <= 1e-14 * np.abs(u).max()
+
+
+@pytest.mark.parametrize("d,p", [(2, 3), (3, 2)])
+def test_restart_round_trip_and_refusals(tmp_path, d, p):
+    L = 2 * math.pi
+    mesh = dg.structured_mesh(d, (3,) * d, (0,) * d, (L,) * d, p_g=p, amp=0.05, seed=1)
+    u = (dg.ic_isentropic_vortex(mesh, p) if d == 2 else dg.ic_taylor_green(mesh, p, 0.3))
+    u = u + 1e-3 * dg.randn(4, u.shape)
+    path = tmp_path / "r.msh"
+    dg.write_restart(path, mesh, p, u)
+    v = dg.read_restart(path, mesh, p)
+    assert np.abs(v - u).max() <= 1e-14 * np.abs(u).max()
+    with pytest.raises(dg._ffi.DgxError):
+        dg.read_restart(path, mesh, p + 1)
+    other = dg.structured_mesh(d, (2,) * d, (0,) * d, (L,) * d)
+    with pytest.raises(dg._ffi.DgxError):
+        dg.read_restart(path, other, p)
+
+
+def _pieces(path):
+    root = ET.parse(path).getroot()
+    assert root.tag == "VTKFile" and root.get("type") == "UnstructuredGrid"
+    grid = root.find("UnstructuredGrid")
+    return grid.findall("Piece")
+
+
+def _array(node, name=None):
+    for da in node.iter("DataArray"):
+        if da.get("Name") == name:
+            return np.array([float(v) for v in da.text.split()])
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("d,p", [(2, 2), (2, 3), (3, 2)])
+def test_vtu_solution_at_plotting_nodes(tmp_path, d, p):
+    n = (3, 2) if d == 2 else (2, 2, 2)
+    hi = (3.0, 2.0) if d == 2 else (2.0, 2.0, 2.0)
+    mesh = dg.structured_mesh(d, n, (0,) * d, hi)
+    # every field a polynomial of degree <= 1 per direction: exact at the plotting nodes
+    rho_f = lambda *X: 1.0 + 0.1 * X[0] + 0.05 * X[1]
+    vel_f = [lambda *X: 0.3 + 0.02 * X[1], lambda *X: -0.1 + 0.03 * X[0]] + ([lambda *X: 0.05 * X[2]] if d == 3 else [])
+    # pressure from the conserved variables is not a polynomial; compare it through rho, rho V, E
+    u = _state_from(mesh, p, rho_f, vel_f, lambda *X: 1.0 + 0.02 * X[0])
+    path = tmp_path / "s.vtu"
+    dg.write_vtu(path, mesh, p, u)
+    pieces = _pieces(path)
+    assert len(pieces) == mesh.n_elem
+    n1 = p + 1
+    r = -1.0 + 2.0 * np.arange(n1) / p
+    for e, pc in enumerate(pieces):
+        assert int(pc.get("NumberOfPoints")) == n1 ** d and int(pc.get("NumberOfCells")) == p ** d
+        pts = np.array([float(v) for v in pc.find("Points").find("DataArray").text.split()]).reshape(-1, 3)
+        lo = mesh.xyz_g[e].min(axis=1)
+        hi_e = mesh.xyz_g[e].max(axis=1)
+        grid = np.stack(np.meshgrid(*([r] * d), indexing="ij"), axis=-1).reshape(-1, d)[:, ::-1]   # r fastest
+        expect = lo + (grid + 1.0) / 2.0 * (hi_e - lo)
+        assert np.allclose(pts[:, :d], expect, atol=2e-7)
+        X = [expect[:, k] for k in range(d)]
+        rho = _array(pc, "$\\rho$")
+        assert np.allclose(rho, rho_f(*X), rtol=1e-7)
+        vel = _array(pc, "$\\mathbf{v}$").reshape(-1, 3)
+        for k in range(d):
+            assert np.allclose(vel[:, k], vel_f[k](*X), rtol=1e-7, atol=1e-8)
+        pr = _array(pc, "$\\mathit{p}$")
+        ent = _array(pc, "$\\mathit{s}$")
+        mach = _array(pc, "mach")
+        assert np.allclose(ent, np.log(pr * rho ** -GAMMA), rtol=1e-6, atol=1e-7)
+        assert np.allclose(mach, np.sqrt((vel[:, :d] ** 2).sum(1) * rho / (GAMMA * pr)), rtol=1e-6)
+        assert np.all(_array(pc, "index") == e) and np.all(_array(pc, "p_ref") == p)
+        conn = _array(pc, "connectivity").astype(int)
+        assert conn.size == p ** d * 2 ** d and conn.max() == n1 ** d - 1
