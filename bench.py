@@ -269,7 +269,7 @@ def bench_implicit(args, dg, torch):
                             "jvp_per_call": jvp_ms, "precond_apply_per_call": apply_ms,
                             "jvp_calls": timed.jvp_calls, "gmres_jvp_calls": gmres_jvps},
            "precond_apply": {"bytes": blk_bytes, "achieved": blk_bytes / (apply_ms * 1e-3) / 1e9,
-                             "unit": "GB/s", "kernel": "torch.bmm f32 (batched GEMV)"},
+                             "unit": "GB/s", "kernel": "dgx::block_apply_kernel (FP32 blocks, FP64 vectors)"},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                         "kernel": "dgx::lin_kernel<3,3,1,false> (+ layout and trace kernels)"}}

@@ -72,6 +72,8 @@ SIGNATURES = {
     "dgx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "dgx_block_inverse": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "dgx_block_apply": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p]),
     "dgx_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "dgx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "dgx_get_stream": (ctypes.c_void_p, [ctypes.c_void_p]),

@@ -179,6 +179,32 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const double* __res
         singular[blk] = flag;
 }
 
+// y_b = A_b x_b for n x n FP32 blocks stored column-major (A_b[j][i] at col[b][j * n + i]), FP64 vectors:
+// thread i accumulates row i over the columns, so each load instruction of a wavefront reads 64
+// consecutive floats of one column; x_b is staged in LDS and read as a broadcast.
+__global__ __launch_bounds__(256) void block_apply_kernel(const float* __restrict__ col, int n,
+                                                          const double* __restrict__ x, double* __restrict__ y)
+{
+    __shared__ float sx[320];
+    const int64_t blk = blockIdx.x;
+    const float* A = col + blk * (int64_t)n * n;
+    for (int j = threadIdx.x; j < n; j += 256)
+        sx[j] = (float)x[blk * n + j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) {
+        float acc[4] = { 0.f, 0.f, 0.f, 0.f };
+        int j = 0;
+        for (; j + 4 <= n; j += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                acc[u] = fmaf(A[(int64_t)(j + u) * n + i], sx[j + u], acc[u]);
+        }
+        for (; j < n; ++j)
+            acc[0] = fmaf(A[(int64_t)j * n + i], sx[j], acc[0]);
+        y[blk * n + i] = (double)((acc[0] + acc[1]) + (acc[2] + acc[3]));
+    }
+}
+
 template <int R>
 hipError_t launch(int64_t nb, int n, const double* B, const double* shift, float* out, int32_t* singular,
                   hipStream_t st)
@@ -216,4 +242,18 @@ extern "C" dgx_status dgx_block_inverse(int64_t n_blocks, int n, const double* b
     default: e = dgx::launch<10>(n_blocks, n, blocks, shift, inv, singular, st); break;
     }
     return e == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+}
+
+extern "C" dgx_status dgx_block_apply(int64_t n_blocks, int n, const float* blocks_colmajor, const double* x,
+                                      double* y, void* stream)
+{
+    if (n_blocks < 0 || n_blocks > 0x7fffffff || n < 1 || n > 320)
+        return DGX_ERR_INVALID_ARG;
+    if (n_blocks == 0)
+        return DGX_OK;
+    if (!blocks_colmajor || !x || !y)
+        return DGX_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(dgx::block_apply_kernel, dim3((unsigned)n_blocks), dim3(256), 0, (hipStream_t)stream,
+                       blocks_colmajor, n, x, y);
+    return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
 }

@@ -172,12 +172,12 @@ class BlockJacobi:
     def apply(self, y, transpose=False):
         import torch
         E, n = self.Pinv.shape[0], self.Pinv.shape[1]
-        if transpose:
-            if self.PinvT is None:
-                self.PinvT = self.Pinv.transpose(1, 2).contiguous()
-            P = self.PinvT
-        else:
-            P = self.Pinv
+        if self.PinvT is None:
+            self.PinvT = self.Pinv.transpose(1, 2).contiguous()
+        if self.Pinv.is_cuda and self.dtype == torch.float32 and n <= 320 and y.dtype == torch.float64:
+            # dgx_block_apply reads column-major blocks: Pinv^T row-major for Pinv, Pinv itself for Pinv^T
+            return block_apply(self.Pinv if transpose else self.PinvT, y)
+        P = self.PinvT if transpose else self.Pinv
         return torch.bmm(P, y.reshape(E, n, 1).to(self.dtype)).reshape(-1).to(y.dtype)
 
 
@@ -205,6 +205,26 @@ def block_inverse(B, shift):
         b = int(bad[0, 0])
         raise FloatingPointError(f"block {b} is singular (no pivot in column {int(sing[b]) - 1})")
     return out
+
+
+def block_apply(A_colmajor, x):
+    """y_b = A_b x_b on the GPU (dgx_block_apply): A_colmajor float32 cuda [nb][n][n] holding each A_b
+    column-major (i.e. A_b^T row-major), x float64 cuda with nb * n values.  Returns float64, flat."""
+    import ctypes
+    import torch
+    from . import _ffi
+    nb, n = A_colmajor.shape[0], A_colmajor.shape[1]
+    if not (A_colmajor.is_cuda and A_colmajor.dtype == torch.float32 and A_colmajor.is_contiguous()):
+        raise ValueError("blocks must be a contiguous float32 cuda tensor [nb][n][n]")
+    x = x.reshape(-1)
+    if not (x.is_cuda and x.dtype == torch.float64 and x.is_contiguous() and x.numel() == nb * n):
+        raise ValueError("x must be a contiguous float64 cuda tensor of nb * n values")
+    y = torch.empty_like(x)
+    lib = _ffi.load()
+    _ffi.check(lib.dgx_block_apply(nb, n, ctypes.c_void_p(A_colmajor.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                   ctypes.c_void_p(y.data_ptr()),
+                                   ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)))
+    return y
 
 
 def solve_linear(solver, b, inv_dt=0.0, transpose=False, precond=None, rtol=1e-4, restart=60, max_iter=600):

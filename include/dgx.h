@@ -138,6 +138,14 @@ dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const
 dgx_status dgx_block_inverse(int64_t n_blocks, int n, const double* blocks, const double* shift, float* inv,
                              int32_t* singular, void* stream);
 
+/* Block-diagonal product y[b] = A[b] x[b] for n_blocks FP32 n x n blocks (n <= 320) stored column-major
+ * (the row-major inverse of dgx_block_inverse read as A^T, or its transpose copied row-major read as
+ * A), FP64 x and y, device pointers, launched on `stream` without synchronising.  The preconditioner
+ * application of the matrix-free Krylov solver; no reference counterpart (PETSc applies its ILU
+ * factors inside KSPSolve, solve_implicit.c:404). */
+dgx_status dgx_block_apply(int64_t n_blocks, int n, const float* blocks_colmajor, const double* x, double* y,
+                           void* stream);
+
 /* Time of the last dgx_step / dgx_rk_stage kernel work in milliseconds, from HIP events recorded on
  * the context stream around the launches (0 when unavailable). */
 double dgx_last_elapsed_ms(const dgx_ctx* ctx);
