@@ -817,6 +817,35 @@ dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const
     return lin_apply(ctx, false, apply_minv, layout, ptr_kind, w, out, "dgx_jvp");
 }
 
+dgx_status dgx_jacobian_blocks(dgx_ctx* ctx, int apply_minv, double* blocks_t)
+{
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    dgx_ctx* c = ctx;
+    if (!blocks_t)
+        return fail(c, DGX_ERR_INVALID_ARG, "dgx_jacobian_blocks: bad arguments");
+    if (c->visc)
+        return fail(c, DGX_ERR_INVALID_ARG, "dgx_jacobian_blocks: the linearized residual is built for Euler");
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    dgx::LinArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.u = c->state_buf ? c->up : c->u;
+    a.met = c->met;
+    a.imass = c->imass;
+    a.tr_u = c->tr[c->cur];
+    a.nbr = c->nbr;
+    a.slot_elem = c->hex3 ? c->slot_elem : nullptr;
+    a.n_elem = c->Ep;
+    a.apply_minv = apply_minv ? 1 : 0;
+    HIP_TRY(c, dgx::launch_lin_block(c->d, c->p, c->flux, a, blocks_t, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess)
+        c->last_ms = ms;
+    return DGX_OK;
+}
+
 double dgx_last_elapsed_ms(const dgx_ctx* ctx)
 {
     return ctx ? ctx->last_ms : 0.0;

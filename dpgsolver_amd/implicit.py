@@ -150,23 +150,36 @@ class BlockJacobi:
         self.Pinv = None
 
     def update(self, inv_dt):
-        """Probe the blocks at the solver's current state and invert diag(inv_dt) - B per volume."""
+        """The blocks B at the solver's current state (one dgx_jacobian_blocks launch on the GPU; JVP
+        probes otherwise) and the inverses of diag(inv_dt) - B per volume."""
         import torch
         E, nv, ns = self.solver.shape
-        B = block_diagonal(self.solver, self.colors, self.n_colors, self.device)
-        idt = torch.as_tensor(inv_dt, dtype=torch.float64, device=self.device).reshape(-1)
         n = nv * ns
-        if B.is_cuda and n <= 320 and self.dtype == torch.float32:
-            self.Pinv = block_inverse(B, idt.expand(E).contiguous())
+        idt = torch.as_tensor(inv_dt, dtype=torch.float64, device=self.device).reshape(-1)
+        direct = torch.device(self.device).type == "cuda" and hasattr(self.solver, "jacobian_blocks")
+        if direct and n <= 320 and self.dtype == torch.float32:
+            BT = torch.empty((E, n, n), dtype=torch.float64, device=self.device)
+            self.solver.jacobian_blocks(BT, apply_minv=True)
+            # (shift I - B^T)^-1 = ((shift I - B)^-1)^T: the transposed inverse, which is also the
+            # column-major form the forward application reads
+            self.PinvT = block_inverse(BT, idt.expand(E).contiguous())
+            del BT
+            self.Pinv = self.PinvT.transpose(1, 2).contiguous()
+            return
+        if direct:
+            B = torch.empty((E, n, n), dtype=torch.float64, device=self.device)
+            self.solver.jacobian_blocks(B, apply_minv=True)
+            B = B.transpose(1, 2).contiguous()
         else:
-            # larger blocks (HEX p=4), other precisions and the CPU: the solver library, in batches of
-            # 64 blocks (its batched LU fails to allocate its workspace from 128 blocks of 320 x 320 up
-            # on MI355X with ROCm 7)
-            B.neg_()
-            B.diagonal(dim1=1, dim2=2).add_(idt.view(-1, 1) if idt.numel() > 1 else idt)
-            self.Pinv = torch.empty(B.shape, dtype=self.dtype, device=self.device)
-            for i in range(0, E, 64):
-                self.Pinv[i:i + 64] = torch.linalg.inv(B[i:i + 64])
+            B = block_diagonal(self.solver, self.colors, self.n_colors, self.device)
+        # larger blocks (HEX p=4), other precisions and the CPU: the solver library, in batches of 64
+        # blocks (its batched LU fails to allocate its workspace from 128 blocks of 320 x 320 up on
+        # MI355X with ROCm 7)
+        B.neg_()
+        B.diagonal(dim1=1, dim2=2).add_(idt.view(-1, 1) if idt.numel() > 1 else idt)
+        self.Pinv = torch.empty(B.shape, dtype=self.dtype, device=self.device)
+        for i in range(0, E, 64):
+            self.Pinv[i:i + 64] = torch.linalg.inv(B[i:i + 64])
         self.PinvT = None
 
     def apply(self, y, transpose=False):

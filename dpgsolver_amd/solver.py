@@ -142,6 +142,18 @@ class Solver:
         del keep_in, keep_out
         return out
 
+    def jacobian_blocks(self, out, apply_minv=True):
+        """Transposed volume-diagonal blocks of J (of M^-1 R with apply_minv) at the current state into
+        out, a float64 cuda tensor [E][nv*Ns][nv*Ns]: out[e][j][i] = dR_i/du_j (dgx_jacobian_blocks)."""
+        n = self.nv * self.ns
+        import torch
+        if not (isinstance(out, torch.Tensor) and out.is_cuda and out.dtype == torch.float64 and out.is_contiguous()
+                and tuple(out.shape) == (self.n_elem, n, n)):
+            raise ValueError(f"out must be a contiguous float64 cuda tensor of shape {(self.n_elem, n, n)}")
+        torch.cuda.synchronize()
+        self._check(self._lib.dgx_jacobian_blocks(self._ctx, int(bool(apply_minv)), ctypes.c_void_p(out.data_ptr())))
+        return out
+
     def sync(self):
         elem = ctypes.c_int64(-1)
         st = self._lib.dgx_sync(self._ctx, ctypes.byref(elem))

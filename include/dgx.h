@@ -128,6 +128,15 @@ dgx_status dgx_sync(dgx_ctx* ctx, int64_t* elem);
 dgx_status dgx_vjp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* v, double* out);
 dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const double* w, double* out);
 
+/* Volume-diagonal blocks of the linearization at the current state (Euler): for every volume e and
+ * unit direction j = v*Ns + n of that volume (neighbours' directions zero), blocks_t[e][j][i] =
+ * d(R or M^-1 R)_i / d u_j with i = v'*Ns + n' (the transposed block, row-major; device pointer of
+ * n_elem * (nv*Ns)^2 doubles, volumes in list order).  One launch; equals the JVPs of the unit
+ * directions restricted to their volume.  Replaces the assembly of the volume-diagonal entries of the
+ * reference's implicit matrix (compute_rlhs_dg with the lhs, dg/solve_dg.c:138-145) for a block
+ * preconditioner. */
+dgx_status dgx_jacobian_blocks(dgx_ctx* ctx, int apply_minv, double* blocks_t);
+
 /* Batched block inversion for a block-Jacobi preconditioner: inv[b] = (shift[b] I - blocks[b])^-1 for
  * n_blocks dense n x n blocks (n <= 320), blocks row-major FP64, inv row-major FP32, all device
  * pointers; singular[b] = 0, or k + 1 when column k had no non-zero pivot (the block's output is then
