@@ -179,4 +179,8 @@ def test_navier_stokes_refused():
     s.set_state(u)
     with pytest.raises(dg._ffi.DgxError, match="Euler"):
         dg.solve_implicit(s, mesh, 2, max_steps=2)
+    import torch
+    E, nv, ns = s.shape
+    with pytest.raises(dg._ffi.DgxError, match="Euler"):
+        s.jacobian_blocks(torch.empty((E, nv * ns, nv * ns), dtype=torch.float64, device="cuda"))
     s.close()

@@ -10,8 +10,9 @@
 - write -> read returns the state to 1e-14 (16 significant digits, Bezier round trip); a file for
   another order or volume count is refused;
 - the VTU file parses as XML with one Piece per volume; points are the equally spaced plotting nodes;
-  a state polynomial of degree <= p is reproduced exactly at them (density, velocity, pressure) and the
-  entropy log(p rho^-gamma) and Mach number follow from them (2-D and 3-D).
+  density and momentum polynomial of degree <= p per direction are reproduced exactly at them (density,
+  velocity), and the entropy log(p rho^-gamma) and Mach number follow from the written pressure (2-D and
+  3-D); bad arguments, missing, truncated and coefficient-less files are refused.
 """
 import math
 import os
@@ -142,7 +143,7 @@ def test_vtu_solution_at_plotting_nodes(tmp_path, d, p):
     n = (3, 2) if d == 2 else (2, 2, 2)
     hi = (3.0, 2.0) if d == 2 else (2.0, 2.0, 2.0)
     mesh = dg.structured_mesh(d, n, (0,) * d, hi)
-    # every field a polynomial of degree <= 1 per direction: exact at the plotting nodes
+    # rho and V linear, so rho and rho V have degree <= 2 <= p per direction: exact at the plotting nodes
     rho_f = lambda *X: 1.0 + 0.1 * X[0] + 0.05 * X[1]
     vel_f = [lambda *X: 0.3 + 0.02 * X[1], lambda *X: -0.1 + 0.03 * X[0]] + ([lambda *X: 0.05 * X[2]] if d == 3 else [])
     # pressure from the conserved variables is not a polynomial; compare it through rho, rho V, E
@@ -175,3 +176,25 @@ def test_vtu_solution_at_plotting_nodes(tmp_path, d, p):
         assert np.all(_array(pc, "index") == e) and np.all(_array(pc, "p_ref") == p)
         conn = _array(pc, "connectivity").astype(int)
         assert conn.size == p ** d * 2 ** d and conn.max() == n1 ** d - 1
+
+
+def test_malformed_and_bad_arguments(tmp_path):
+    mesh = dg.structured_mesh(2, (2, 2), (0, 0), (1, 1))
+    u = dg.ic_isentropic_vortex(mesh, 2)
+    path = tmp_path / "r.msh"
+    dg.write_restart(path, mesh, 2, u)
+    text = open(path).read()
+    cut = tmp_path / "cut.msh"
+    open(cut, "w").write(text[: len(text) // 2 + text[len(text) // 2:].index("\n")] + "\n")   # truncated
+    with pytest.raises(dg._ffi.DgxError):
+        dg.read_restart(cut, mesh, 2)
+    nosol = tmp_path / "nosol.msh"
+    open(nosol, "w").write(text[: text.index("$SolutionCoefficients")])
+    with pytest.raises(dg._ffi.DgxError):
+        dg.read_restart(nosol, mesh, 2)
+    with pytest.raises(dg._ffi.DgxError):
+        dg.read_restart(tmp_path / "missing.msh", mesh, 2)
+    with pytest.raises(dg._ffi.DgxError):
+        dg.write_vtu(tmp_path / "s.vtu", mesh, 0, u)
+    with pytest.raises(dg._ffi.DgxError):
+        dg.write_restart(tmp_path / "no_such_dir" / "r.msh", mesh, 2, u)
