@@ -8,6 +8,7 @@
 // here as a CSR list on the host: integer bookkeeping over the face list).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -31,36 +32,30 @@ struct GeoOps {
     double Igm[NMAX], Igp[NMAX];        // l_m at -1 / +1
 };
 
-__device__ inline void digits(int a, int n, int* out)
+// d_j f at a geometry node: the line through it along j starts at `base` with stride `st`, the node is
+// number `i` on that line
+__device__ inline double grad_line(const GeoOps& g, const double* f, int i, int base, int st)
 {
-    out[0] = a % n;
-    out[1] = (a / n) % n;
-    out[2] = a / (n * n);
-}
-
-// d_j f at geometry node a (f on the ng^D grid in LDS)
-template <int D>
-__device__ inline double grad_g(const GeoOps& g, const double* f, int j, int a)
-{
-    int ia[3];
-    digits(a, g.ng, ia);
-    int stride = 1;
-    for (int k = 0; k < j; ++k)
-        stride *= g.ng;
-    const int base = a - ia[j] * stride;
     double s = 0.0;
     for (int m = 0; m < g.ng; ++m)
-        s = fma(g.Dg[ia[j] * NMAX + m], f[base + m * stride], s);
+        s = fma(g.Dg[i * NMAX + m], f[base + m * st], s);
     return s;
 }
 
-// Wc[(w * Ng + a) * Nc + n]: weight of geometry node a at cubature node n, w = 0 interpolation,
-// w = 1 + j the derivative along j; Wf[(lf * Ng + a) * Nf + q]: interpolation to the face nodes of lf
-// (tangential directions in order).  Products of the 1-D operators, built once on the host; a-major, so
-// that consecutive threads (nodes) read consecutive weights.
-template <int D>
+// one 1-D pass of the sum-factorised interpolation: out[p] = sum_m A[i][m] in[base + m * st], A = n_out x ng
+__device__ inline double pass_line(const double* A, const double* in, int i, int base, int st, int nin)
+{
+    double s = 0.0;
+    for (int m = 0; m < nin; ++m)
+        s = fma(A[i * NMAX + m], in[base + m * st], s);
+    return s;
+}
+
+// Wf[(lf * Ng + a) * Nf + q]: interpolation weight of geometry node a at face node q of local face lf
+// (tangential directions in order), products of the 1-D operators built once on the host, a-major so
+// that consecutive threads (face nodes) read consecutive weights.
+template <int D, int VM>
 __global__ __launch_bounds__(BLOCK) void geometry_kernel(GeoOps gk, const double* __restrict__ xyz_g, int64_t E,
-                                                         const double* __restrict__ Wc,
                                                          const double* __restrict__ Wf,
                                                          const int64_t* __restrict__ ffirst,
                                                          const int64_t* __restrict__ flist,
@@ -69,102 +64,80 @@ __global__ __launch_bounds__(BLOCK) void geometry_kernel(GeoOps gk, const double
                                                          double* __restrict__ nrm, double* __restrict__ jdf,
                                                          int* __restrict__ bad)
 {
-    __shared__ double sX[3][VMAX];
-    __shared__ double sJ[9][VMAX];
-    __shared__ double sV[3][VMAX];
-    __shared__ double sM[9][VMAX];
+    // geometry-node fields, then the two intermediate grids of the sum-factorised interpolation: the
+    // pass-s output (18 fields) reuses sM and sX, which are no longer read once pass r is complete
+    __shared__ double sbuf[33][VM];     // VM: the largest of the geometry and cubature grids
+    double(*sX)[VM] = sbuf;             // [3]  element-local coordinates
+    double(*sJ)[VM] = sbuf + 3;         // [9]  d_j X_i at the geometry nodes; pass-r output (with sV)
+    double(*sV)[VM] = sbuf + 12;        // [6]  curl products; pass-r output
+    double(*sM)[VM] = sbuf + 18;        // [9]  metric terms at the geometry nodes
+    // pass-s output slot k: sM for k < 9, sX for 9 <= k < 12, the last 6 rows for 12 <= k < 18
+#define SW(k) sbuf[(k) < 9 ? 18 + (k) : ((k) < 12 ? (k) - 9 : 15 + (k))]
     __shared__ GeoOps g;                // the 1-D operators, read with per-thread indices
     const int64_t e = blockIdx.x;
     const int t = threadIdx.x;
     if (t == 0)
         g = gk;
     __syncthreads();
-    const int Ng = gk.Ng, Nc = gk.Nc, Nf = gk.Nf;
+    const int ng = gk.ng, nc = gk.nc, Ng = gk.Ng, Nc = gk.Nc, Nf = gk.Nf;
     const double* Xe = xyz_g + e * D * Ng;
 
+    // thread t owns geometry node t (Ng <= BLOCK): its position on the three lines through it
+    const bool gnode = t < Ng;
+    int ia[3] = { t % ng, (t / ng) % ng, D == 3 ? t / (ng * ng) : 0 };
+    const int gst[3] = { 1, ng, ng * ng };
+    int gbase[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        gbase[j] = t - ia[j] * gst[j];
+
     // element-local coordinates: the curl form's products X_a dX_b at the element's own scale
-    for (int q = t; q < Ng; q += BLOCK)
+    if (gnode)
         for (int i = 0; i < D; ++i)
-            sX[i][q] = Xe[i * Ng + q] - Xe[i * Ng];
+            sX[i][t] = Xe[i * Ng + t] - Xe[i * Ng];
     __syncthreads();
-    for (int q = t; q < Ng; q += BLOCK)
+    if (gnode)
         for (int i = 0; i < D; ++i)
             for (int j = 0; j < D; ++j)
-                sJ[i * 3 + j][q] = grad_g<D>(g, sX[i], j, q);
+                sJ[i * 3 + j][t] = grad_line(g, sX[i], ia[j], gbase[j], gst[j]);
     __syncthreads();
 
     if constexpr (D == 2) {
-        for (int q = t; q < Ng; q += BLOCK) {
-            sM[0 + 2 * 0][q] = sJ[1 * 3 + 1][q];        //  y_s
-            sM[0 + 2 * 1][q] = -sJ[1 * 3 + 0][q];       // -y_r
-            sM[1 + 2 * 0][q] = -sJ[0 * 3 + 1][q];       // -x_s
-            sM[1 + 2 * 1][q] = sJ[0 * 3 + 0][q];        //  x_r
+        if (gnode) {
+            sM[0 + 2 * 0][t] = sJ[1 * 3 + 1][t];        //  y_s
+            sM[0 + 2 * 1][t] = -sJ[1 * 3 + 0][t];       // -y_r
+            sM[1 + 2 * 0][t] = -sJ[0 * 3 + 1][t];       // -x_s
+            sM[1 + 2 * 1][t] = sJ[0 * 3 + 0][t];        //  x_r
         }
     } else {
         // C_ic = 1/2 [curl(X_a grad X_b) - curl(X_b grad X_a)]_c, (a, b) = (i+1, i+2) mod 3
-        for (int q = t; q < Ng; q += BLOCK)
+        if (gnode)
             for (int c = 0; c < 9; ++c)
-                sM[c][q] = 0.0;
+                sM[c][t] = 0.0;
         for (int i = 0; i < 3; ++i) {
-            const int ia = (i + 1) % 3, ib = (i + 2) % 3;
+            const int ia_ = (i + 1) % 3, ib_ = (i + 2) % 3;
             for (int s = 0; s < 2; ++s) {
-                const int a = s ? ib : ia, b = s ? ia : ib;
+                const int a = s ? ib_ : ia_, b = s ? ia_ : ib_;
                 __syncthreads();                        // sV free (previous curl done)
-                for (int q = t; q < Ng; q += BLOCK)
+                if (gnode)
                     for (int k = 0; k < 3; ++k)
-                        sV[k][q] = sX[a][q] * sJ[b * 3 + k][q];
+                        sV[k][t] = sX[a][t] * sJ[b * 3 + k][t];
                 __syncthreads();
                 const double sg = s ? -0.5 : 0.5;
-                for (int q = t; q < Ng; q += BLOCK)
+                if (gnode)
                     for (int c = 0; c < 3; ++c) {
                         const int k1 = (c + 1) % 3, k2 = (c + 2) % 3;
-                        const double curl = grad_g<D>(g, sV[k2], k1, q) - grad_g<D>(g, sV[k1], k2, q);
-                        sM[i + 3 * c][q] += sg * curl;
+                        const double curl = grad_line(g, sV[k2], ia[k1], gbase[k1], gst[k1]) -
+                                            grad_line(g, sV[k1], ia[k2], gbase[k2], gst[k2]);
+                        sM[i + 3 * c][t] += sg * curl;
                     }
             }
         }
     }
     __syncthreads();
 
-    for (int n = t; n < Nc; n += BLOCK) {
-        double m[D * D], J[D][D];
-#pragma unroll
-        for (int c = 0; c < D * D; ++c)
-            m[c] = 0.0;
-#pragma unroll
-        for (int i = 0; i < D; ++i)
-#pragma unroll
-            for (int j = 0; j < D; ++j)
-                J[i][j] = 0.0;
-        for (int a = 0; a < Ng; ++a) {
-            const double w0 = Wc[(int64_t)a * Nc + n];
-#pragma unroll
-            for (int c = 0; c < D * D; ++c)
-                m[c] = fma(w0, sM[c][a], m[c]);
-#pragma unroll
-            for (int j = 0; j < D; ++j) {
-                const double wj = Wc[((int64_t)(1 + j) * Ng + a) * Nc + n];
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-                    J[i][j] = fma(wj, sX[i][a], J[i][j]);
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < D * D; ++c)
-            met[(e * D * D + c) * Nc + n] = m[c];
-        double det;
-        if constexpr (D == 2)
-            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        else
-            det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                  J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-        jdet[e * Nc + n] = det;
-        if (!(det > 0.0))
-            atomicOr(bad, 1);
-    }
-
-    // faces whose left volume is e: thread t serves face ffirst[e] + t / Nf, node t % Nf
+    // faces whose left volume is e, from the metric terms at the geometry nodes: thread t serves face
+    // ffirst[e] + t / Nf, node t % Nf
     const int64_t f0 = ffirst[e], nfe = ffirst[e + 1] - f0;
     for (int64_t u = t; u < nfe * Nf; u += BLOCK) {
         const int64_t f = flist[f0 + u / Nf];
@@ -194,6 +167,97 @@ __global__ __launch_bounds__(BLOCK) void geometry_kernel(GeoOps gk, const double
         for (int i = 0; i < D; ++i)
             nrm[(f * Nf + q) * D + i] = v[i] / sn;
     }
+
+    // cubature values by sum factorisation (r, then s, then t, as the host build): the metric terms
+    // interpolated, d_j X_i with the derivative factor in direction j
+    double J[3][3];
+    if constexpr (D == 2) {
+        // pass r on the (nc, ng) grid: sJ[0..3] = I_r M_c, sV[0..1] = I_r X_i, sV[2..3] = D_r X_i
+        const bool p1 = t < nc * ng;
+        const int ir = t % nc, as = t / nc;
+        if (p1) {
+            for (int c = 0; c < 4; ++c)
+                sJ[c][t] = pass_line(g.Igc, sM[c], ir, ng * as, 1, ng);
+            for (int i = 0; i < 2; ++i) {
+                sV[i][t] = pass_line(g.Igc, sX[i], ir, ng * as, 1, ng);
+                sV[2 + i][t] = pass_line(g.Dgc, sX[i], ir, ng * as, 1, ng);
+            }
+        }
+        __syncthreads();
+        if (t < Nc) {
+            const int jr = t % nc, js = t / nc;
+            for (int c = 0; c < 4; ++c)
+                met[(e * 4 + c) * Nc + t] = pass_line(g.Igc, sJ[c], js, jr, nc, ng);
+            for (int i = 0; i < 2; ++i) {
+                J[i][0] = pass_line(g.Igc, sV[2 + i], js, jr, nc, ng);
+                J[i][1] = pass_line(g.Dgc, sV[i], js, jr, nc, ng);
+            }
+        }
+    } else {
+        // pass r on the (nc, ng, ng) grid: sJ[c] = I_r M_c, sV[i] = I_r X_i, sV[3 + i] = D_r X_i
+        const bool p1 = t < nc * ng * ng;
+        const int ir = t % nc, rest1 = t / nc;              // rest1 = as + ng * at
+        if (p1) {
+            for (int c = 0; c < 9; ++c)
+                sJ[c][t] = pass_line(g.Igc, sM[c], ir, ng * rest1, 1, ng);
+            for (int i = 0; i < 3; ++i) {
+                sV[i][t] = pass_line(g.Igc, sX[i], ir, ng * rest1, 1, ng);
+                sV[3 + i][t] = pass_line(g.Dgc, sX[i], ir, ng * rest1, 1, ng);
+            }
+        }
+        __syncthreads();
+        // pass s on the (nc, nc, ng) grid: SW(c) = I_s (I_r M_c); SW(9+i) = I_s I_r X_i, SW(12+i) = D_s I_r X_i,
+        // SW(15+i) = I_s D_r X_i
+        const bool p2 = t < nc * nc * ng;
+        const int jr = t % nc, js = (t / nc) % nc, at = t / (nc * nc);
+        if (p2) {
+            const int base = jr + nc * ng * at;             // (jr, m, at) on the pass-r grid: + nc * m
+            for (int c = 0; c < 9; ++c)
+                SW(c)[t] = pass_line(g.Igc, sJ[c], js, base, nc, ng);
+            for (int i = 0; i < 3; ++i) {
+                SW(9 + i)[t] = pass_line(g.Igc, sV[i], js, base, nc, ng);
+                SW(12 + i)[t] = pass_line(g.Dgc, sV[i], js, base, nc, ng);
+                SW(15 + i)[t] = pass_line(g.Igc, sV[3 + i], js, base, nc, ng);
+            }
+        }
+        __syncthreads();
+        if (t < Nc) {
+            const int kt = t / (nc * nc), base = t - kt * nc * nc;   // (jr, js, m) on the pass-s grid: + nc^2 m
+            for (int c = 0; c < 9; ++c)
+                met[(e * 9 + c) * Nc + t] = pass_line(g.Igc, SW(c), kt, base, nc * nc, ng);
+            for (int i = 0; i < 3; ++i) {
+                J[i][0] = pass_line(g.Igc, SW(15 + i), kt, base, nc * nc, ng);
+                J[i][1] = pass_line(g.Igc, SW(12 + i), kt, base, nc * nc, ng);
+                J[i][2] = pass_line(g.Dgc, SW(9 + i), kt, base, nc * nc, ng);
+            }
+        }
+    }
+    if (t < Nc) {
+        double det;
+        if constexpr (D == 2)
+            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        else
+            det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                  J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        jdet[e * Nc + t] = det;
+        if (!(det > 0.0))
+            atomicOr(bad, 1);
+    }
+#undef SW
+}
+
+template <int D>
+void launch_geometry(bool small, const GeoOps& g, const double* x, int64_t E, const double* wf, const int64_t* first,
+                     const int64_t* list, const int8_t* lf, double* om, double* oj, double* on, double* of, int* bad,
+                     hipStream_t st)
+{
+    if (small)
+        hipLaunchKernelGGL((geometry_kernel<D, 64>), dim3((unsigned)E), dim3(BLOCK), 0, st, g, x, E, wf, first, list,
+                           lf, om, oj, on, of, bad);
+    else
+        hipLaunchKernelGGL((geometry_kernel<D, VMAX>), dim3((unsigned)E), dim3(BLOCK), 0, st, g, x, E, wf, first,
+                           list, lf, om, oj, on, of, bad);
 }
 
 struct DevBuf {
@@ -249,19 +313,9 @@ extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int pt
         dgx_lagrange_1d(ng, xg, 1, &p1, 0, g.Igp);
     }
 
-    // weight tables (see geometry_kernel)
-    const int Ng = g.Ng, Nc = g.Nc, Nf = g.Nf, ng = g.ng, nc = g.nc;
-    std::vector<double> wc((size_t)(d + 1) * Ng * Nc), wf((size_t)2 * d * Ng * Nf);
-    for (int w = 0; w <= d; ++w)
-        for (int a = 0; a < Ng; ++a)
-            for (int n = 0; n < Nc; ++n) {
-                const int ia[3] = { a % ng, (a / ng) % ng, a / (ng * ng) };
-                const int in[3] = { n % nc, (n / nc) % nc, n / (nc * nc) };
-                double v = 1.0;
-                for (int k = 0; k < d; ++k)
-                    v *= (w == 1 + k ? g.Dgc : g.Igc)[in[k] * NMAX + ia[k]];
-                wc[((size_t)w * Ng + a) * Nc + n] = v;
-            }
+    // face weight table (see geometry_kernel)
+    const int Ng = g.Ng, Nf = g.Nf, ng = g.ng, nc = g.nc;
+    std::vector<double> wf((size_t)2 * d * Ng * Nf);
     for (int lf = 0; lf < 2 * d; ++lf)
         for (int a = 0; a < Ng; ++a)
             for (int q = 0; q < Nf; ++q) {
@@ -302,7 +356,7 @@ extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int pt
     const size_t b_met = sizeof(double) * (size_t)E * d * d * g.Nc, b_jd = sizeof(double) * (size_t)E * g.Nc;
     const size_t b_nrm = sizeof(double) * (size_t)(F > 0 ? F : 1) * g.Nf * d;
     const size_t b_jdf = sizeof(double) * (size_t)(F > 0 ? F : 1) * g.Nf;
-    DevBuf dx, dwc, dwf, dfirst, dlist, dlf, dbad, dmet, djd, dnrm, djdf;
+    DevBuf dx, dwf, dfirst, dlist, dlf, dbad, dmet, djd, dnrm, djdf;
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     dgx_status rc = DGX_OK;
@@ -318,7 +372,6 @@ extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int pt
     GEO_TRY(hipEventCreate(&ev0));
     GEO_TRY(hipEventCreate(&ev1));
     GEO_TRY(hipMalloc(&dx.p, b_xyz));
-    GEO_TRY(hipMalloc(&dwc.p, sizeof(double) * wc.size()));
     GEO_TRY(hipMalloc(&dwf.p, sizeof(double) * wf.size()));
     GEO_TRY(hipMalloc(&dfirst.p, sizeof(int64_t) * first.size()));
     GEO_TRY(hipMalloc(&dlist.p, sizeof(int64_t) * list.size()));
@@ -336,21 +389,21 @@ extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int pt
         double* on = ptr_kind == DGX_PTR_HOST ? (double*)dnrm.p : normals_fc;
         double* of = ptr_kind == DGX_PTR_HOST ? (double*)djdf.p : jdet_fc;
         GEO_TRY(hipMemcpyAsync(dx.p, mesh->xyz_g, b_xyz, hipMemcpyHostToDevice, st));
-        GEO_TRY(hipMemcpyAsync(dwc.p, wc.data(), sizeof(double) * wc.size(), hipMemcpyHostToDevice, st));
         GEO_TRY(hipMemcpyAsync(dwf.p, wf.data(), sizeof(double) * wf.size(), hipMemcpyHostToDevice, st));
         GEO_TRY(hipMemcpyAsync(dfirst.p, first.data(), sizeof(int64_t) * first.size(), hipMemcpyHostToDevice, st));
         GEO_TRY(hipMemcpyAsync(dlist.p, list.data(), sizeof(int64_t) * list.size(), hipMemcpyHostToDevice, st));
         GEO_TRY(hipMemcpyAsync(dlf.p, lfs.data(), lfs.size(), hipMemcpyHostToDevice, st));
         GEO_TRY(hipMemcpyAsync(dbad.p, &bad, sizeof(int), hipMemcpyHostToDevice, st));
         GEO_TRY(hipEventRecord(ev0, st));
+        // grids of at most 64 points (orders up to 3 in 3-D, 7 in 2-D) use the smaller LDS footprint
+        const int vm = std::max(g.Ng, std::max(g.Nc, g.nc * g.ng * (d == 3 ? g.ng : 1)));
+        const bool small = vm <= 64;
         if (d == 2)
-            hipLaunchKernelGGL(geometry_kernel<2>, dim3((unsigned)E), dim3(BLOCK), 0, st, g, (const double*)dx.p, E,
-                               (const double*)dwc.p, (const double*)dwf.p, (const int64_t*)dfirst.p,
-                               (const int64_t*)dlist.p, (const int8_t*)dlf.p, om, oj, on, of, (int*)dbad.p);
+            launch_geometry<2>(small, g, (const double*)dx.p, E, (const double*)dwf.p, (const int64_t*)dfirst.p,
+                                (const int64_t*)dlist.p, (const int8_t*)dlf.p, om, oj, on, of, (int*)dbad.p, st);
         else
-            hipLaunchKernelGGL(geometry_kernel<3>, dim3((unsigned)E), dim3(BLOCK), 0, st, g, (const double*)dx.p, E,
-                               (const double*)dwc.p, (const double*)dwf.p, (const int64_t*)dfirst.p,
-                               (const int64_t*)dlist.p, (const int8_t*)dlf.p, om, oj, on, of, (int*)dbad.p);
+            launch_geometry<3>(small, g, (const double*)dx.p, E, (const double*)dwf.p, (const int64_t*)dfirst.p,
+                                (const int64_t*)dlist.p, (const int8_t*)dlf.p, om, oj, on, of, (int*)dbad.p, st);
         GEO_TRY(hipGetLastError());
         GEO_TRY(hipEventRecord(ev1, st));
         if (ptr_kind == DGX_PTR_HOST) {
