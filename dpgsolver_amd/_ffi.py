@@ -71,6 +71,9 @@ SIGNATURES = {
                                   ctypes.POINTER(DgxGeometry), ctypes.POINTER(ctypes.c_void_p)]),
     "dgx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "dgx_jacobian_blocks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "dgx_jacobian_blocks_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "dgx_block_inverse_f32": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dgx_block_inverse": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dgx_block_apply": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,

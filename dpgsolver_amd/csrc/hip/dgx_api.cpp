@@ -817,12 +817,9 @@ dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const
     return lin_apply(ctx, false, apply_minv, layout, ptr_kind, w, out, "dgx_jvp");
 }
 
-dgx_status dgx_jacobian_blocks(dgx_ctx* ctx, int apply_minv, double* blocks_t)
+static dgx_status jacobian_blocks(dgx_ctx* c, int apply_minv, double* bt64, float* bt32)
 {
-    if (!ctx)
-        return DGX_ERR_INVALID_ARG;
-    dgx_ctx* c = ctx;
-    if (!blocks_t)
+    if (!bt64 && !bt32)
         return fail(c, DGX_ERR_INVALID_ARG, "dgx_jacobian_blocks: bad arguments");
     if (c->visc)
         return fail(c, DGX_ERR_INVALID_ARG, "dgx_jacobian_blocks: the linearized residual is built for Euler");
@@ -837,13 +834,30 @@ dgx_status dgx_jacobian_blocks(dgx_ctx* ctx, int apply_minv, double* blocks_t)
     a.slot_elem = c->hex3 ? c->slot_elem : nullptr;
     a.n_elem = c->Ep;
     a.apply_minv = apply_minv ? 1 : 0;
-    HIP_TRY(c, dgx::launch_lin_block(c->d, c->p, c->flux, a, blocks_t, c->stream));
+    if (bt64)
+        HIP_TRY(c, dgx::launch_lin_block(c->d, c->p, c->flux, a, bt64, c->stream));
+    else
+        HIP_TRY(c, dgx::launch_lin_block_f32(c->d, c->p, c->flux, a, bt32, c->stream));
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess)
         c->last_ms = ms;
     return DGX_OK;
+}
+
+dgx_status dgx_jacobian_blocks(dgx_ctx* ctx, int apply_minv, double* blocks_t)
+{
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    return jacobian_blocks(ctx, apply_minv, blocks_t, nullptr);
+}
+
+dgx_status dgx_jacobian_blocks_f32(dgx_ctx* ctx, int apply_minv, float* blocks_t)
+{
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    return jacobian_blocks(ctx, apply_minv, nullptr, blocks_t);
 }
 
 double dgx_last_elapsed_ms(const dgx_ctx* ctx)

@@ -20,11 +20,10 @@ namespace {
 
 constexpr int TG = 32;
 
-template <int R>
-__global__ __launch_bounds__(1024) void block_inverse_kernel(const double* __restrict__ B,
-                                                             const double* __restrict__ shift, int n,
-                                                             float* __restrict__ out,
-                                                             int32_t* __restrict__ singular)
+// TIN = float allows out == B (in place): every block reads all its entries before any is written.
+template <int R, class TIN>
+__global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const double* __restrict__ shift, int n,
+                                                             float* out, int32_t* __restrict__ singular)
 {
     __shared__ float s_prow[2][TG * R];
     __shared__ float s_pcol[2][TG * R];
@@ -36,7 +35,7 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const double* __res
     const int tj = threadIdx.x & (TG - 1);
     const int ti = threadIdx.x >> 5;
     const int64_t blk = blockIdx.x;
-    const double* Bb = B + blk * (int64_t)n * n;
+    const TIN* Bb = B + blk * (int64_t)n * n;
     const double sh = shift[blk];
 
     float a[R][R];
@@ -46,7 +45,7 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const double* __res
 #pragma unroll
         for (int c = 0; c < R; ++c) {
             const int j = tj + TG * c;
-            a[r][c] = (i < n && j < n) ? (float)((i == j ? sh : 0.0) - Bb[(int64_t)i * n + j]) : 0.f;
+            a[r][c] = (i < n && j < n) ? (float)((i == j ? sh : 0.0) - (double)Bb[(int64_t)i * n + j]) : 0.f;
         }
     }
     if (threadIdx.x < TG * R) {     // defined permutation entries even when a singular block repeats a row
@@ -205,20 +204,18 @@ __global__ __launch_bounds__(256) void block_apply_kernel(const float* __restric
     }
 }
 
-template <int R>
-hipError_t launch(int64_t nb, int n, const double* B, const double* shift, float* out, int32_t* singular,
+template <int R, class TIN>
+hipError_t launch(int64_t nb, int n, const TIN* B, const double* shift, float* out, int32_t* singular,
                   hipStream_t st)
 {
-    hipLaunchKernelGGL(block_inverse_kernel<R>, dim3((unsigned)nb), dim3(1024), 0, st, B, shift, n, out,
+    hipLaunchKernelGGL((block_inverse_kernel<R, TIN>), dim3((unsigned)nb), dim3(1024), 0, st, B, shift, n, out,
                        singular);
     return hipGetLastError();
 }
 
-}  // namespace
-}  // namespace dgx
-
-extern "C" dgx_status dgx_block_inverse(int64_t n_blocks, int n, const double* blocks, const double* shift,
-                                        float* inv, int32_t* singular, void* stream)
+template <class TIN>
+dgx_status block_inverse(int64_t n_blocks, int n, const TIN* blocks, const double* shift, float* inv,
+                         int32_t* singular, void* stream)
 {
     if (n_blocks < 0 || n_blocks > 0x7fffffff || n < 1 || n > 320)
         return DGX_ERR_INVALID_ARG;
@@ -230,18 +227,33 @@ extern "C" dgx_status dgx_block_inverse(int64_t n_blocks, int n, const double* b
     const int R = (n + 31) / 32;
     hipError_t e;
     switch (R) {
-    case 1: e = dgx::launch<1>(n_blocks, n, blocks, shift, inv, singular, st); break;
-    case 2: e = dgx::launch<2>(n_blocks, n, blocks, shift, inv, singular, st); break;
-    case 3: e = dgx::launch<3>(n_blocks, n, blocks, shift, inv, singular, st); break;
-    case 4: e = dgx::launch<4>(n_blocks, n, blocks, shift, inv, singular, st); break;
-    case 5: e = dgx::launch<5>(n_blocks, n, blocks, shift, inv, singular, st); break;
-    case 6: e = dgx::launch<6>(n_blocks, n, blocks, shift, inv, singular, st); break;
-    case 7: e = dgx::launch<7>(n_blocks, n, blocks, shift, inv, singular, st); break;
-    case 8: e = dgx::launch<8>(n_blocks, n, blocks, shift, inv, singular, st); break;
-    case 9: e = dgx::launch<9>(n_blocks, n, blocks, shift, inv, singular, st); break;
-    default: e = dgx::launch<10>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    case 1: e = launch<1>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    case 2: e = launch<2>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    case 3: e = launch<3>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    case 4: e = launch<4>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    case 5: e = launch<5>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    case 6: e = launch<6>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    case 7: e = launch<7>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    case 8: e = launch<8>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    case 9: e = launch<9>(n_blocks, n, blocks, shift, inv, singular, st); break;
+    default: e = launch<10>(n_blocks, n, blocks, shift, inv, singular, st); break;
     }
     return e == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+}
+
+}  // namespace
+}  // namespace dgx
+
+extern "C" dgx_status dgx_block_inverse(int64_t n_blocks, int n, const double* blocks, const double* shift,
+                                        float* inv, int32_t* singular, void* stream)
+{
+    return dgx::block_inverse(n_blocks, n, blocks, shift, inv, singular, stream);
+}
+
+extern "C" dgx_status dgx_block_inverse_f32(int64_t n_blocks, int n, const float* blocks, const double* shift,
+                                            float* inv, int32_t* singular, void* stream)
+{
+    return dgx::block_inverse(n_blocks, n, blocks, shift, inv, singular, stream);
 }
 
 extern "C" dgx_status dgx_block_apply(int64_t n_blocks, int n, const float* blocks_colmajor, const double* x,

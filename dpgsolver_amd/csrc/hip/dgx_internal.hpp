@@ -22,6 +22,7 @@ hipError_t launch_hex3_trace(const double* u, double* tr, const int32_t* slot_el
 hipError_t launch_reduce_max(const double* bm, int64_t nb, double* out, unsigned int* nonfinite, hipStream_t st);
 hipError_t launch_lin(int d, int p, int flux, bool adjoint, const LinArgs& args, hipStream_t st);
 hipError_t launch_lin_block(int d, int p, int flux, const LinArgs& args, double* bt, hipStream_t st);
+hipError_t launch_lin_block_f32(int d, int p, int flux, const LinArgs& args, float* bt, hipStream_t st);
 hipError_t launch_scale_by_imass(const double* x, const double* imass, double* y, int64_t E, int nv, int ns,
                                  hipStream_t st);
 hipError_t launch_ns_stage(int d, int p, int flux, const StageArgs& args, double* gtr, hipStream_t st);

@@ -147,7 +147,7 @@ class BlockJacobi:
         self.solver, self.device = solver, device
         self.dtype = torch.float32 if dtype is None else dtype
         self.colors, self.n_colors = element_colors(mesh)
-        self.Pinv = None
+        self.Pinv = self.PinvT = None
 
     def update(self, inv_dt):
         """The blocks B at the solver's current state (one dgx_jacobian_blocks launch on the GPU; JVP
@@ -158,13 +158,13 @@ class BlockJacobi:
         idt = torch.as_tensor(inv_dt, dtype=torch.float64, device=self.device).reshape(-1)
         direct = torch.device(self.device).type == "cuda" and hasattr(self.solver, "jacobian_blocks")
         if direct and n <= 320 and self.dtype == torch.float32:
-            BT = torch.empty((E, n, n), dtype=torch.float64, device=self.device)
+            # one FP32 buffer: the transposed blocks, inverted in place.  (shift I - B^T)^-1 =
+            # ((shift I - B)^-1)^T is the column-major form the forward application reads; the row-major
+            # inverse for transposed solves is made on first use.
+            self.Pinv = self.PinvT = None
+            BT = torch.empty((E, n, n), dtype=torch.float32, device=self.device)
             self.solver.jacobian_blocks(BT, apply_minv=True)
-            # (shift I - B^T)^-1 = ((shift I - B)^-1)^T: the transposed inverse, which is also the
-            # column-major form the forward application reads
-            self.PinvT = block_inverse(BT, idt.expand(E).contiguous())
-            del BT
-            self.Pinv = self.PinvT.transpose(1, 2).contiguous()
+            self.PinvT = block_inverse(BT, idt.expand(E).contiguous(), out=BT)
             return
         if direct:
             B = torch.empty((E, n, n), dtype=torch.float64, device=self.device)
@@ -184,34 +184,43 @@ class BlockJacobi:
 
     def apply(self, y, transpose=False):
         import torch
-        E, n = self.Pinv.shape[0], self.Pinv.shape[1]
         if self.PinvT is None:
             self.PinvT = self.Pinv.transpose(1, 2).contiguous()
-        if self.Pinv.is_cuda and self.dtype == torch.float32 and n <= 320 and y.dtype == torch.float64:
+        if transpose and self.Pinv is None:
+            self.Pinv = self.PinvT.transpose(1, 2).contiguous()
+        E, n = self.PinvT.shape[0], self.PinvT.shape[1]
+        if self.PinvT.is_cuda and self.dtype == torch.float32 and n <= 320 and y.dtype == torch.float64:
             # dgx_block_apply reads column-major blocks: Pinv^T row-major for Pinv, Pinv itself for Pinv^T
             return block_apply(self.Pinv if transpose else self.PinvT, y)
+        if self.Pinv is None:
+            self.Pinv = self.PinvT.transpose(1, 2).contiguous()
         P = self.PinvT if transpose else self.Pinv
         return torch.bmm(P, y.reshape(E, n, 1).to(self.dtype)).reshape(-1).to(y.dtype)
 
 
-def block_inverse(B, shift):
-    """(shift_b I - B_b)^-1 for every block on the GPU (dgx_block_inverse: FP32 Gauss-Jordan with partial
-    pivoting, one workgroup per block, n <= 320).  B: float64 cuda tensor [nb][n][n]; shift: float64
-    cuda tensor [nb].  Returns float32 [nb][n][n]; raises if a block is singular."""
+def block_inverse(B, shift, out=None):
+    """(shift_b I - B_b)^-1 for every block on the GPU (dgx_block_inverse / dgx_block_inverse_f32: FP32
+    Gauss-Jordan with partial pivoting, one workgroup per block, n <= 320).  B: float64 or float32 cuda
+    tensor [nb][n][n]; shift: float64 cuda tensor [nb]; out: float32 [nb][n][n] (may be B when B is
+    float32).  Returns out; raises if a block is singular."""
     import ctypes
     import torch
     from . import _ffi
     nb, n = B.shape[0], B.shape[1]
-    if not (B.is_cuda and B.dtype == torch.float64 and B.is_contiguous() and B.shape == (nb, n, n)):
-        raise ValueError("B must be a contiguous float64 cuda tensor [nb][n][n]")
+    if not (B.is_cuda and B.dtype in (torch.float64, torch.float32) and B.is_contiguous() and B.shape == (nb, n, n)):
+        raise ValueError("B must be a contiguous float64/float32 cuda tensor [nb][n][n]")
     if not (shift.is_cuda and shift.dtype == torch.float64 and shift.is_contiguous() and shift.numel() == nb):
         raise ValueError("shift must be a contiguous float64 cuda tensor of nb values")
-    out = torch.empty((nb, n, n), dtype=torch.float32, device=B.device)
+    if out is None:
+        out = torch.empty((nb, n, n), dtype=torch.float32, device=B.device)
+    elif not (out.is_cuda and out.dtype == torch.float32 and out.is_contiguous() and out.shape == (nb, n, n)):
+        raise ValueError("out must be a contiguous float32 cuda tensor [nb][n][n]")
     sing = torch.empty(nb, dtype=torch.int32, device=B.device)
     lib = _ffi.load()
-    st = lib.dgx_block_inverse(nb, n, ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(shift.data_ptr()),
-                               ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(sing.data_ptr()),
-                               ctypes.c_void_p(torch.cuda.current_stream(B.device).cuda_stream))
+    fn = lib.dgx_block_inverse if B.dtype == torch.float64 else lib.dgx_block_inverse_f32
+    st = fn(nb, n, ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(shift.data_ptr()),
+            ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(sing.data_ptr()),
+            ctypes.c_void_p(torch.cuda.current_stream(B.device).cuda_stream))
     _ffi.check(st)
     bad = torch.nonzero(sing)
     if bad.numel():

@@ -144,14 +144,16 @@ class Solver:
 
     def jacobian_blocks(self, out, apply_minv=True):
         """Transposed volume-diagonal blocks of J (of M^-1 R with apply_minv) at the current state into
-        out, a float64 cuda tensor [E][nv*Ns][nv*Ns]: out[e][j][i] = dR_i/du_j (dgx_jacobian_blocks)."""
+        out, a float64 or float32 cuda tensor [E][nv*Ns][nv*Ns]: out[e][j][i] = dR_i/du_j
+        (dgx_jacobian_blocks / dgx_jacobian_blocks_f32)."""
         n = self.nv * self.ns
         import torch
-        if not (isinstance(out, torch.Tensor) and out.is_cuda and out.dtype == torch.float64 and out.is_contiguous()
-                and tuple(out.shape) == (self.n_elem, n, n)):
-            raise ValueError(f"out must be a contiguous float64 cuda tensor of shape {(self.n_elem, n, n)}")
+        if not (isinstance(out, torch.Tensor) and out.is_cuda and out.dtype in (torch.float64, torch.float32)
+                and out.is_contiguous() and tuple(out.shape) == (self.n_elem, n, n)):
+            raise ValueError(f"out must be a contiguous float64/float32 cuda tensor of shape {(self.n_elem, n, n)}")
         torch.cuda.synchronize()
-        self._check(self._lib.dgx_jacobian_blocks(self._ctx, int(bool(apply_minv)), ctypes.c_void_p(out.data_ptr())))
+        fn = self._lib.dgx_jacobian_blocks if out.dtype == torch.float64 else self._lib.dgx_jacobian_blocks_f32
+        self._check(fn(self._ctx, int(bool(apply_minv)), ctypes.c_void_p(out.data_ptr())))
         return out
 
     def sync(self):

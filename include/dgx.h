@@ -136,6 +136,8 @@ dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const
  * reference's implicit matrix (compute_rlhs_dg with the lhs, dg/solve_dg.c:138-145) for a block
  * preconditioner. */
 dgx_status dgx_jacobian_blocks(dgx_ctx* ctx, int apply_minv, double* blocks_t);
+/* The same blocks rounded to FP32 (half the memory: 107 GB instead of 215 GB for 64^3 HEX p=3). */
+dgx_status dgx_jacobian_blocks_f32(dgx_ctx* ctx, int apply_minv, float* blocks_t);
 
 /* Batched block inversion for a block-Jacobi preconditioner: inv[b] = (shift[b] I - blocks[b])^-1 for
  * n_blocks dense n x n blocks (n <= 320), blocks row-major FP64, inv row-major FP32, all device
@@ -146,6 +148,9 @@ dgx_status dgx_jacobian_blocks(dgx_ctx* ctx, int apply_minv, double* blocks_t);
  * which a matrix-free solver cannot factor; the blocks come from dgx_jvp probes. */
 dgx_status dgx_block_inverse(int64_t n_blocks, int n, const double* blocks, const double* shift, float* inv,
                              int32_t* singular, void* stream);
+/* The same from FP32 blocks; inv may equal blocks (inversion in place). */
+dgx_status dgx_block_inverse_f32(int64_t n_blocks, int n, const float* blocks, const double* shift, float* inv,
+                                 int32_t* singular, void* stream);
 
 /* Block-diagonal product y[b] = A[b] x[b] for n_blocks FP32 n x n blocks (n <= 320) stored column-major
  * (the row-major inverse of dgx_block_inverse read as A^T, or its transpose copied row-major read as

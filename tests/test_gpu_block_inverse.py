@@ -8,7 +8,8 @@
 - a singular block is reported (and its neighbours in the batch are unaffected); n > 320 and bad sizes
   are refused; zero blocks is a no-op;
 - dgx_block_apply (FP32 column-major blocks times FP64 vectors) equals numpy's products of the same FP32
-  blocks to FP32 accumulation accuracy, for A and A^T, across the tile edges.
+  blocks to FP32 accumulation accuracy, for A and A^T, across the tile edges;
+- dgx_block_inverse_f32 from FP32 blocks, in place, equals the inverse of the same FP32-rounded blocks.
 """
 import ctypes
 
@@ -134,3 +135,22 @@ def test_block_apply_against_numpy(n):
     scale = np.abs(A64).max() * np.abs(xb).max() * n
     assert np.abs(y_col - ref).max() <= 1e-6 * scale
     assert np.abs(y_colT - refT).max() <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("n", [9, 64, 320])
+def test_block_inverse_f32_in_place(n):
+    _require_gpu()
+    import torch
+    rng = np.random.default_rng(900 + n)
+    nb = 19
+    B = (rng.standard_normal((nb, n, n)) / np.sqrt(n)).astype(np.float32)
+    shift = 1.5 + rng.random(nb)
+    Bt = torch.from_numpy(B).cuda()
+    st = torch.from_numpy(shift).cuda()
+    out = implicit.block_inverse(Bt, st, out=Bt)
+    assert out.data_ptr() == Bt.data_ptr()
+    X = out.cpu().numpy().astype(np.float64)
+    for b in range(nb):
+        A = shift[b] * np.eye(n) - B[b].astype(np.float64)
+        ref = np.linalg.inv(A)
+        assert np.linalg.norm(X[b] - ref) / np.linalg.norm(ref) <= 2e-6 * np.linalg.cond(A)

@@ -169,6 +169,9 @@ def test_jacobian_blocks_equal_probed_blocks(d, p, n, flux, minv):
     B = implicit.block_diagonal(_Minv(), col, nc, "cuda")
     diff = (BT.transpose(1, 2) - B).abs().max().item()
     assert diff <= 1e-12 * B.abs().max().item(), diff
+    BT32 = torch.empty(BT.shape, dtype=torch.float32, device="cuda")
+    s.jacobian_blocks(BT32, apply_minv=minv)
+    assert torch.equal(BT32, BT.to(torch.float32))
     s.close()
 
 
