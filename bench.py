@@ -185,9 +185,10 @@ def bench_vjp(args, dg, torch):
 def bench_implicit(args, dg, torch):
     """Pseudo-transient Newton-Krylov (dpgsolver_amd.implicit) on a 3-D Euler HEX p=3 n^3 periodic mesh,
     perturbed M=2 uniform flow, Roe-Pike: `--newton-steps` Newton steps from the same initial state, each
-    = the volume-diagonal blocks (dgx_jacobian_blocks) + their inversion (dgx_block_inverse) + GMRES(60)
-    to rtol 1e-4 (at most 600 iterations).  One unit = one GMRES iteration (one JVP, one preconditioner
-    application, CGS2 over the basis)."""
+    = the volume-diagonal blocks (dgx_jacobian_blocks_f32) + their inversion in place
+    (dgx_block_inverse_f32) + GMRES(60) to rtol 1e-4 (at most 600 iterations) preconditioned by block
+    Jacobi (`--nk-sweep gauss-seidel`: one multicolour block Gauss-Seidel sweep).  One unit = one GMRES
+    iteration (one JVP, one preconditioner application, CGS2 over the basis)."""
     from dpgsolver_amd import implicit
     n, p = args.nk_n, 3
     mesh = dg.structured_mesh(3, (n, n, n), (0, 0, 0), (2 * math.pi,) * 3, p_g=1)
@@ -241,7 +242,7 @@ def bench_implicit(args, dg, torch):
 
     def run(steps, timed):
         s.set_state(torch.from_numpy(u0).cuda())
-        pc = TimedPrec(timed, mesh, "cuda")
+        pc = TimedPrec(timed, mesh, "cuda", sweep=args.nk_sweep)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _, hist = implicit.solve_implicit(timed, mesh, p, exit_tol=0.0, max_steps=steps, precond=pc)
@@ -265,7 +266,7 @@ def bench_implicit(args, dg, torch):
            "scaling": "weak", "vs_baseline": None, "dtype": "f64 (f32 preconditioner blocks)",
            "data": "synthetic",
            "config": {"workload": f"implicit Newton-GMRES {n}^3 HEX p=3 periodic [0,2pi]^3, M=2 uniform flow + "
-                                  f"1e-2 perturbation, Roe-Pike, block-Jacobi (f32), GMRES(60) rtol 1e-4",
+                                  f"1e-2 perturbation, Roe-Pike, block {args.nk_sweep} (f32), GMRES(60) rtol 1e-4",
                       "elements": mesh.n_elem, "dof": dof, "newton_steps": args.newton_steps,
                       "gmres_iterations": its, "max_rhs": [h[0] for h in hist]},
            "breakdown_ms": {"newton_step": wall / args.newton_steps * 1e3,
@@ -369,6 +370,7 @@ def main():
     ap.add_argument("--geo-n", type=int, default=64)
     ap.add_argument("--nk-n", type=int, default=16)
     ap.add_argument("--newton-steps", type=int, default=4)
+    ap.add_argument("--nk-sweep", default="jacobi", choices=["jacobi", "gauss-seidel"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

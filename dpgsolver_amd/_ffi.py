@@ -78,6 +78,8 @@ SIGNATURES = {
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dgx_block_apply": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "dgx_block_apply_sel": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dgx_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "dgx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "dgx_get_stream": (ctypes.c_void_p, [ctypes.c_void_p]),

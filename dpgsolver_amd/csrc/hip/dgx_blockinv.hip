@@ -182,10 +182,11 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const
 // thread i accumulates row i over the columns, so each load instruction of a wavefront reads 64
 // consecutive floats of one column; x_b is staged in LDS and read as a broadcast.
 __global__ __launch_bounds__(256) void block_apply_kernel(const float* __restrict__ col, int n,
-                                                          const double* __restrict__ x, double* __restrict__ y)
+                                                          const double* __restrict__ x, double* __restrict__ y,
+                                                          const int32_t* __restrict__ sel)
 {
     __shared__ float sx[320];
-    const int64_t blk = blockIdx.x;
+    const int64_t blk = sel ? sel[blockIdx.x] : blockIdx.x;
     const float* A = col + blk * (int64_t)n * n;
     for (int j = threadIdx.x; j < n; j += 256)
         sx[j] = (float)x[blk * n + j];
@@ -266,6 +267,20 @@ extern "C" dgx_status dgx_block_apply(int64_t n_blocks, int n, const float* bloc
     if (!blocks_colmajor || !x || !y)
         return DGX_ERR_INVALID_ARG;
     hipLaunchKernelGGL(dgx::block_apply_kernel, dim3((unsigned)n_blocks), dim3(256), 0, (hipStream_t)stream,
-                       blocks_colmajor, n, x, y);
+                       blocks_colmajor, n, x, y, (const int32_t*)nullptr);
+    return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+}
+
+extern "C" dgx_status dgx_block_apply_sel(int64_t n_sel, const int32_t* sel, int n, const float* blocks_colmajor,
+                                          const double* x, double* y, void* stream)
+{
+    if (n_sel < 0 || n_sel > 0x7fffffff || n < 1 || n > 320)
+        return DGX_ERR_INVALID_ARG;
+    if (n_sel == 0)
+        return DGX_OK;
+    if (!sel || !blocks_colmajor || !x || !y)
+        return DGX_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(dgx::block_apply_kernel, dim3((unsigned)n_sel), dim3(256), 0, (hipStream_t)stream,
+                       blocks_colmajor, n, x, y, sel);
     return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
 }

@@ -135,18 +135,25 @@ def block_diagonal(solver, colors, n_colors, device):
 
 
 class BlockJacobi:
-    """Block-Jacobi preconditioner of A = diag(inv_dt) - M^-1 dR/du on the volume-diagonal blocks.
+    """Block-Jacobi preconditioner of A = diag(inv_dt) - M^-1 dR/du on the volume-diagonal blocks, or,
+    with sweep="gauss-seidel", one multicolour block Gauss-Seidel sweep over the volume colouring (each
+    colour's blocks solved with the coupling to the colours already swept, obtained by one JVP of the
+    partial solution: the matrix-free counterpart of an incomplete factorisation's lower triangle).
 
     The inverted blocks are kept in `dtype` (float32 by default): applying them is a batched GEMV that
     reads every block once per Krylov iteration, the largest HBM stream of the iteration, and the
     preconditioner's precision does not limit the solution's (GMRES measures its residual on A itself,
     with the FP64 JVP)."""
 
-    def __init__(self, solver, mesh, device="cuda", dtype=None):
+    def __init__(self, solver, mesh, device="cuda", dtype=None, sweep="jacobi"):
         import torch
-        self.solver, self.device = solver, device
+        if sweep not in ("jacobi", "gauss-seidel"):
+            raise ValueError(f"unknown sweep {sweep!r}")
+        self.solver, self.device, self.sweep = solver, device, sweep
         self.dtype = torch.float32 if dtype is None else dtype
         self.colors, self.n_colors = element_colors(mesh)
+        self.sel = [torch.from_numpy(np.nonzero(self.colors == c)[0].astype(np.int32)).to(device)
+                    for c in range(self.n_colors)]
         self.Pinv = self.PinvT = None
 
     def update(self, inv_dt):
@@ -184,6 +191,8 @@ class BlockJacobi:
 
     def apply(self, y, transpose=False):
         import torch
+        if self.sweep == "gauss-seidel":
+            return self._gauss_seidel(y, transpose)
         if self.PinvT is None:
             self.PinvT = self.Pinv.transpose(1, 2).contiguous()
         if transpose and self.Pinv is None:
@@ -196,6 +205,54 @@ class BlockJacobi:
             self.Pinv = self.PinvT.transpose(1, 2).contiguous()
         P = self.PinvT if transpose else self.Pinv
         return torch.bmm(P, y.reshape(E, n, 1).to(self.dtype)).reshape(-1).to(y.dtype)
+
+    def _gauss_seidel(self, y, transpose):
+        """x_c = A_cc^-1 (y_c - sum over swept colours c' of A_cc' x_c'), colours in order (A^T: reverse
+        order, transposed blocks, couplings by VJP).  A_cc' = -(M^-1 J)_cc' off the diagonal blocks, so the
+        coupling of the partial solution x (zero on the colours not yet swept) is + (M^-1 J x)_c."""
+        import torch
+        if self.PinvT is None:
+            self.PinvT = self.Pinv.transpose(1, 2).contiguous()
+        if transpose and self.Pinv is None:
+            self.Pinv = self.PinvT.transpose(1, 2).contiguous()
+        shape = self.solver.shape
+        A_col = self.Pinv if transpose else self.PinvT
+        lin = self.solver.vjp if transpose else self.solver.jvp
+        order = range(self.n_colors - 1, -1, -1) if transpose else range(self.n_colors)
+        y = y.reshape(-1)
+        x = torch.zeros_like(y)
+        jx = torch.empty(shape, dtype=torch.float64, device=y.device)
+        sync = _sync_fn(y.device)
+        for k, c in enumerate(order):
+            if k == 0:
+                rhs = y
+            else:
+                sync()
+                lin(x.view(shape), apply_minv=True, out=jx)
+                rhs = y + jx.reshape(-1)
+            block_apply_sel(A_col, self.sel[c], rhs, x)
+        return x
+
+
+def block_apply_sel(A_colmajor, sel, x, y):
+    """y_b = A_b x_b for the blocks b in sel (dgx_block_apply_sel); the other entries of y are kept."""
+    import ctypes
+    import torch
+    from . import _ffi
+    nb, n = A_colmajor.shape[0], A_colmajor.shape[1]
+    if not (A_colmajor.is_cuda and A_colmajor.dtype == torch.float32 and A_colmajor.is_contiguous()):
+        raise ValueError("blocks must be a contiguous float32 cuda tensor [nb][n][n]")
+    for v in (x, y):
+        if not (v.is_cuda and v.dtype == torch.float64 and v.is_contiguous() and v.numel() == nb * n):
+            raise ValueError("x and y must be contiguous float64 cuda tensors of nb * n values")
+    if not (sel.is_cuda and sel.dtype == torch.int32 and sel.is_contiguous()):
+        raise ValueError("sel must be a contiguous int32 cuda tensor")
+    lib = _ffi.load()
+    _ffi.check(lib.dgx_block_apply_sel(sel.numel(), ctypes.c_void_p(sel.data_ptr()), n,
+                                       ctypes.c_void_p(A_colmajor.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                       ctypes.c_void_p(y.data_ptr()),
+                                       ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)))
+    return y
 
 
 def block_inverse(B, shift, out=None):
@@ -314,7 +371,7 @@ def max_wave_speed(u, sb, d):
 
 def solve_implicit(solver, mesh, p, cfl_initial=1.0, exit_tol=1e-10, exit_ratio=0.0, max_steps=50,
                    rtol=1e-4, restart=60, max_iter=600, precondition=True, verbose=False, device="cuda",
-                   precond=None):
+                   precond=None, sweep="jacobi"):
     """Pseudo-transient Newton-Krylov from the solver's current state until max |M^-1 R| < exit_tol or
     max_rhs / max_rhs_0 < exit_ratio (check_exit, solve_implicit.c:258-284).
 
@@ -322,7 +379,8 @@ def solve_implicit(solver, mesh, p, cfl_initial=1.0, exit_tol=1e-10, exit_ratio=
     [E][nv][Ns] as a torch tensor on `device`, history) with one entry per step: (max_rhs, cfl, GMRES
     iterations, GMRES relative residual); the last entry holds the final max_rhs.  The solver is left
     at the final state.  `precond`: a BlockJacobi (or an object with its update/apply) to use instead of
-    a new one; precondition=False runs plain GMRES."""
+    a new one (`sweep`: "jacobi" or "gauss-seidel", the latter on the GPU only); precondition=False runs
+    plain GMRES."""
     import torch
     from .problem import ops1d
     dev = torch.device(device)
@@ -334,7 +392,7 @@ def solve_implicit(solver, mesh, p, cfl_initial=1.0, exit_tol=1e-10, exit_ratio=
     if not precondition:
         precond = None
     elif precond is None:
-        precond = BlockJacobi(solver, mesh, dev)
+        precond = BlockJacobi(solver, mesh, dev, sweep=sweep)
     r = torch.empty_like(u)
     hist = []
     max_rhs0 = None

@@ -159,6 +159,10 @@ dgx_status dgx_block_inverse_f32(int64_t n_blocks, int n, const float* blocks, c
  * factors inside KSPSolve, solve_implicit.c:404). */
 dgx_status dgx_block_apply(int64_t n_blocks, int n, const float* blocks_colmajor, const double* x, double* y,
                            void* stream);
+/* The same for the blocks sel[0..n_sel) only (block indices into blocks_colmajor, x and y; the other
+ * entries of y are left untouched): one colour of a multicolour block Gauss-Seidel sweep. */
+dgx_status dgx_block_apply_sel(int64_t n_sel, const int32_t* sel, int n, const float* blocks_colmajor,
+                               const double* x, double* y, void* stream);
 
 /* Time of the last dgx_step / dgx_rk_stage kernel work in milliseconds, from HIP events recorded on
  * the context stream around the launches (0 when unavailable). */

@@ -8,7 +8,8 @@ The reference's implicit DG step solves (M/dt - dR/du) du = R with PETSc on the 
   oracle standing in for the GPU gives 2e-13..3e-12 with cond 500..2800), at GMRES rtol 1e-13, on
   QUAD p=2 (Roe), HEX p=1 (LF, 3 colours) and HEX p=3 (Roe, the brick-slot lin kernels);
 - the transposed solve (dgx_vjp): (diag(1/dt) - J_cs^T) psi = g through the oracle's dense Jacobian, and
-  the dual identity <g, A^-1 b> = <A^-T g, b>;
+  the dual identity <g, A^-1 b> = <A^-T g, b>, with the block Jacobi and the multicolour block
+  Gauss-Seidel preconditioners (the step parity above runs both as well);
 - the pseudo-transient Newton-Krylov loop drives a perturbed uniform flow (periodic, M=2) to
   max |M^-1 R| < 1e-10 (QUAD) or by 1e-6 (HEX), and the oracle's residual of the final state agrees;
 - dgx_jacobian_blocks equals the volume-diagonal blocks probed with dgx_jvp on a colouring (1e-12);
@@ -82,12 +83,14 @@ def test_pseudo_time_step_matches_dense_oracle_solve(d, p, n, flux, amp):
     tol = 1e-12 * np.linalg.cond(A)
     pc = implicit.BlockJacobi(s, mesh, "cuda")
     pc.update(torch.from_numpy(idt).cuda())
-    for precond in (pc, None):
+    gs = implicit.BlockJacobi(s, mesh, "cuda", sweep="gauss-seidel")
+    gs.update(torch.from_numpy(idt).cuda())
+    for precond in (pc, gs, None):
         restart = 300 if precond is not None else u.size          # unpreconditioned: full GMRES
         du, res, its = implicit.solve_linear(s, r, torch.from_numpy(idt).cuda(), precond=precond, rtol=1e-13,
                                              restart=restart, max_iter=3000)
         assert res <= 1e-13
-        assert rel(du.cpu().numpy(), du_ref) <= tol, (precond is not None, its)
+        assert rel(du.cpu().numpy(), du_ref) <= tol, (getattr(precond, "sweep", None), its)
     s.close()
 
 
@@ -100,26 +103,29 @@ def test_transposed_solve_and_dual_identity():
     s.set_state(u)
     idt = _inv_dt(mesh, p, u, 5.0)
     idt_t = torch.from_numpy(idt).cuda()
-    pc = implicit.BlockJacobi(s, mesh, "cuda")
-    pc.update(idt_t)
     b = torch.from_numpy(dg.randn(11, u.shape)).cuda()
     g = torch.from_numpy(dg.randn(12, u.shape)).cuda()
-    x, rx, _ = implicit.solve_linear(s, b, idt_t, precond=pc, rtol=1e-13, restart=300, max_iter=3000)
-    psi, rp, _ = implicit.solve_linear(s, g, idt_t, transpose=True, precond=pc, rtol=1e-13, restart=300,
-                                       max_iter=3000)
-    assert rx <= 1e-13 and rp <= 1e-13
     orc = Oracle(mesh, p, num_flux=1, positivity=False, geometry=geo)
     A = np.diag(np.repeat(idt, u[0].size)) - _dense_jacobian(orc, u)
-    psi_n = psi.cpu().numpy().ravel()
-    assert rel(A.T @ psi_n, g.cpu().numpy().ravel()) <= 1e-12 * np.linalg.cond(A)
-    gx = float(torch.dot(g.reshape(-1), x.reshape(-1)))
-    pb = float(torch.dot(psi.reshape(-1), b.reshape(-1)))
-    assert abs(gx - pb) <= 1e-10 * abs(gx)
+    for sweep in ("jacobi", "gauss-seidel"):
+        pc = implicit.BlockJacobi(s, mesh, "cuda", sweep=sweep)
+        pc.update(idt_t)
+        x, rx, _ = implicit.solve_linear(s, b, idt_t, precond=pc, rtol=1e-13, restart=300, max_iter=3000)
+        psi, rp, _ = implicit.solve_linear(s, g, idt_t, transpose=True, precond=pc, rtol=1e-13, restart=300,
+                                           max_iter=3000)
+        assert rx <= 1e-13 and rp <= 1e-13
+        psi_n = psi.cpu().numpy().ravel()
+        assert rel(A.T @ psi_n, g.cpu().numpy().ravel()) <= 1e-12 * np.linalg.cond(A)
+        gx = float(torch.dot(g.reshape(-1), x.reshape(-1)))
+        pb = float(torch.dot(psi.reshape(-1), b.reshape(-1)))
+        assert abs(gx - pb) <= 1e-10 * abs(gx)
     s.close()
 
 
-@pytest.mark.parametrize("d,p,n,flux,steps,drop", [(2, 2, 8, "roe", 25, None), (3, 2, 4, "roe", 25, 1e-6)])
-def test_newton_krylov_reaches_steady_state(d, p, n, flux, steps, drop):
+@pytest.mark.parametrize("d,p,n,flux,steps,drop,sweep", [(2, 2, 8, "roe", 25, None, "jacobi"),
+                                                         (3, 2, 4, "roe", 25, 1e-6, "jacobi"),
+                                                         (2, 2, 8, "roe", 25, None, "gauss-seidel")])
+def test_newton_krylov_reaches_steady_state(d, p, n, flux, steps, drop, sweep):
     """QUAD p=2 reaches the exit tolerance 1e-10 (16 steps with the oracle standing in); on HEX the
     slowly damped modes of a periodic flow (wave vectors nearly normal to the velocity) stall the
     restarted GMRES at large CFL, so the check there is a 1e-6 drop in 25 steps (2.6e-9 after 25 with the
@@ -129,7 +135,7 @@ def test_newton_krylov_reaches_steady_state(d, p, n, flux, steps, drop):
     print(f"implicit solve d={d} p={p} n={n} {flux}", flush=True)
     s = dg.Solver(mesh, geo, p=p, num_flux=flux)
     s.set_state(u)
-    uf, hist = dg.solve_implicit(s, mesh, p, cfl_initial=1.0, exit_tol=1e-10, max_steps=steps)
+    uf, hist = dg.solve_implicit(s, mesh, p, cfl_initial=1.0, exit_tol=1e-10, max_steps=steps, sweep=sweep)
     mx = [h[0] for h in hist]
     print("  max_rhs history", " ".join(f"{v:.1e}" for v in mx), flush=True)
     if drop is None:
