@@ -181,6 +181,34 @@ def test_jacobian_blocks_equal_probed_blocks(d, p, n, flux, minv):
     s.close()
 
 
+def test_jacobian_blocks_past_one_dispatch():
+    """HEX p=3 on 60^3 volumes: 216000 x 320 directions at 64 work-items each is more than one dispatch
+    holds (2^32 work-items), so the blocks go out in several launches.  A uniform state on a uniform
+    periodic mesh makes every volume's block the same up to the rounding of its metric terms; the
+    array starts as NaN, so a volume that no launch reached fails the comparison."""
+    _require_gpu()
+    import torch
+    n, p = 60, 3
+    mesh = dg.structured_mesh(3, (n, n, n), (0, 0, 0), (2 * math.pi,) * 3, p_g=1)
+    geo = dg.geometry(mesh, p)
+    u = dg.ic_uniform_flow(mesh, p, amp=0.0)
+    s = dg.Solver(mesh, geo, p, num_flux="roe")
+    s.set_state(u)
+    E, nv, ns = s.shape
+    assert E * nv * ns * 64 > 2 ** 32
+    print(f"jacobian blocks {E} volumes, {E * (nv * ns) ** 2 * 4 / 2 ** 30:.1f} GiB", flush=True)
+    BT = torch.full((E, nv * ns, nv * ns), float("nan"), dtype=torch.float32, device="cuda")
+    s.jacobian_blocks(BT)
+    torch.cuda.synchronize()
+    b0 = BT[0].clone()
+    tol = 1e-5 * b0.abs().max().item()
+    for e0 in range(0, E, 4096):
+        diff = (BT[e0:e0 + 4096] - b0).abs().amax().item()
+        assert diff <= tol, (e0, diff)             # NaN fails too
+    del BT
+    s.close()
+
+
 def test_navier_stokes_refused():
     _require_gpu()
     mesh, geo, u = _case(2, 2, 4)
