@@ -5,10 +5,17 @@
 // grid of threads and thread (ti, tj) holds the R x R elements (ti + 32 r, tj + 32 c), R = ceil(n/32).
 // In-place Gauss-Jordan with partial pivoting that never moves rows: the pivot of column k is the
 // largest |a_ik| among rows not yet used, found by the 32 threads that hold column k and reduced by
-// every wavefront with cross-lane exchanges; the pivot row and column are broadcast through LDS
-// (double-buffered, two barriers per column).  The element that ends in storage (i, j) belongs to
-// the inverse at (step at which row i was the pivot, pivot row of step j), which the final write
-// applies.  Loads are row-contiguous (32 consecutive doubles per thread row).
+// every wavefront with cross-lane exchanges.  The element that ends in storage (i, j) belongs to the
+// inverse at (step at which row i was the pivot, pivot row of step j), which the final write applies.
+//
+// Blocked by column tiles: the 32 steps of tile c0 (the panel) run on the panel's columns only, with
+// the pivot row and column broadcast through LDS (double-buffered, two barriers per step).  The 32
+// elementary row transformations E_k (each the identity but in column p_k, the pivot row) multiply to
+// T = E_32 ... E_1 = I + W S^T, where S selects the pivot rows p_k and column k of W is the panel's final
+// column k minus e_{p_k} (column k of the panel ends as T e_{p_k}, since E_j e_{p_k} = e_{p_k} for
+// j < k).  Every other column x then becomes T x = x + W x[p], one rank-32 update from the pivot rows'
+// values before the panel (gathered into LDS as the pivots are chosen) and W (written to LDS after the
+// panel): 32 FMAs per element with LDS operands and no per-step selects.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -20,17 +27,197 @@ namespace {
 
 constexpr int TG = 32;
 
+// The 32 (or, for the last tile, n - 32 CK) pivot steps of column tile CK and the update of every other
+// column; CK is a template argument so that every index into the register tile is a constant.
+struct InvShared {
+    float* w;           // [NP][32]
+    float* slab;        // [32][NP]
+    float (*prow)[TG];
+    float* pcol;        // [2][NP]
+    float (*cval)[TG];
+    int (*cidx)[TG];
+    int* piv;
+    int* step;
+};
+
+template <int R, int CK>
+__device__ __forceinline__ void panel(float (&a)[R][R], unsigned& used, int& flag, int n, int ti, int tj,
+                                      const InvShared& sm)
+{
+    constexpr int NP = TG * R;
+    float* const s_w = sm.w;
+    float* const s_slab = sm.slab;
+    float (*const s_prow)[TG] = sm.prow;
+    float (*const s_cval)[TG] = sm.cval;
+    int (*const s_cidx)[TG] = sm.cidx;
+    int* const s_piv = sm.piv;
+    int* const s_step = sm.step;
+    const int k0 = TG * CK;
+    const int nb = (n - k0 < TG) ? n - k0 : TG;      // panel width (>= 1 since R = ceil(n / 32))
+    for (int kk = 0; kk < nb; ++kk) {
+        const int k = k0 + kk;
+        const int buf = k & 1;
+        __syncthreads();
+
+        // every wavefront reduces the 32 candidates (both 32-lane halves hold the same set) as one
+        // 64-bit maximum of (bits of |a|, ~row): non-negative floats order as their bit patterns, and
+        // equal magnitudes go to the smaller row
+        const float cv = s_cval[buf][threadIdx.x & 31];
+        uint64_t key = ((uint64_t)(cv > 0.f ? __float_as_uint(cv) : 0u) << 32) |
+                       (uint32_t)~(uint32_t)s_cidx[buf][threadIdx.x & 31];
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) {
+            const uint64_t k2 = __shfl_xor(key, off);
+            key = k2 > key ? k2 : key;
+        }
+        const float v = __uint_as_float((uint32_t)(key >> 32));
+        int p = (int)~(uint32_t)key;
+        if (!(v > 0.f))
+            flag = k + 1;       // no usable pivot (singular block or non-finite input)
+        p = (p < n) ? p : k;    // keeps every index in range when the block is singular
+
+        if (ti == (p & 31)) {
+            // the row's value as an OR of masked bit patterns: every index into the register tile stays
+            // a constant (a select between tile elements would be turned into a select of addresses)
+            const int rp = p >> 5;
+            unsigned bits = 0u;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                bits |= (r == rp) ? __float_as_uint(a[r][CK]) : 0u;
+            s_prow[buf][tj] = __uint_as_float(bits);
+            used |= 1u << rp;
+        }
+        if (tj == kk) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                sm.pcol[buf * NP + ti + TG * r] = a[r][CK];
+        }
+        if (threadIdx.x == 0) {
+            s_piv[k] = p;
+            s_step[p] = k;
+        }
+        __syncthreads();
+
+        const float apk = s_prow[buf][kk];
+        const float inv = (v > 0.f) ? 1.f / apk : 0.f;
+        const float pj = s_prow[buf][tj];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = ti + TG * r;
+            const float fo = sm.pcol[buf * NP + i] * inv;
+            // pivot row: a_pj (1 - (1 - inv)) = a_pj inv; other rows: a_ij - (a_ik / a_pk) a_pj
+            const float f = (i == p) ? 1.f - inv : fo;
+            a[r][CK] = fmaf(-f, pj, a[r][CK]);
+            if (tj == kk)
+                a[r][CK] = (i == p) ? inv : -fo;
+        }
+
+        // pivot candidates of column k + 1 within the panel (unused rows only)
+        if (kk + 1 < nb && tj == kk + 1) {
+            float best = -1.f;
+            int bi = 0x7fffffff;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = ti + TG * r;
+                const float av = fabsf(a[r][CK]);
+                if (i < n && !((used >> r) & 1u) && av > best) {
+                    best = av;
+                    bi = i;
+                }
+            }
+            s_cval[buf ^ 1][ti] = best;
+            s_cidx[buf ^ 1][ti] = bi;
+        }
+    }
+
+    if (R > 1) {                // a single tile has no column outside the panel
+        // the panel's pivot rows, still as before the panel outside it, by the threads that hold them
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int st = s_step[ti + TG * r] - k0;
+            if (((used >> r) & 1u) && st >= 0 && st < nb) {
+#pragma unroll
+                for (int c = 0; c < R; ++c)
+                    if (c != CK)
+                        s_slab[st * NP + tj + TG * c] = a[r][c];
+            }
+        }
+        // W = panel - e_{p_kk}; columns kk >= nb (past the order) stay zero
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = ti + TG * r;
+            const float wv = (tj < nb) ? a[r][CK] - ((i == s_piv[k0 + tj]) ? 1.f : 0.f) : 0.f;
+            s_w[i * TG + tj] = wv;
+        }
+        __syncthreads();
+
+        // every column outside the panel, earlier tiles included: x += W x[p]
+        for (int kk = 0; kk < nb; ++kk) {
+            float w[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                w[r] = s_w[(ti + TG * r) * TG + kk];
+#pragma unroll
+            for (int c = 0; c < R; ++c) {
+                if (c == CK)
+                    continue;
+                const float x = s_slab[kk * NP + tj + TG * c];
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    a[r][c] = fmaf(w[r], x, a[r][c]);
+            }
+        }
+    }
+
+    if (CK + 1 < R) {
+        // pivot candidates of the next panel's first column
+        if (tj == 0) {
+            float best = -1.f;
+            int bi = 0x7fffffff;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = ti + TG * r;
+                const float av = fabsf(a[r][CK + 1 < R ? CK + 1 : 0]);
+                if (i < n && !((used >> r) & 1u) && av > best) {
+                    best = av;
+                    bi = i;
+                }
+            }
+            s_cval[(k0 + TG) & 1][ti] = best;
+            s_cidx[(k0 + TG) & 1][ti] = bi;
+        }
+    }
+}
+
+template <int R, int CK>
+struct Panels {
+    static __device__ __forceinline__ void run(float (&a)[R][R], unsigned& used, int& flag, int n, int ti, int tj,
+                                               const InvShared& sm)
+    {
+        panel<R, CK>(a, used, flag, n, ti, tj, sm);
+        Panels<R, CK + 1>::run(a, used, flag, n, ti, tj, sm);
+    }
+};
+
+template <int R>
+struct Panels<R, R> {
+    static __device__ __forceinline__ void run(float (&)[R][R], unsigned&, int&, int, int, int, const InvShared&) {}
+};
+
 // TIN = float allows out == B (in place): every block reads all its entries before any is written.
 template <int R, class TIN>
 __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const double* __restrict__ shift, int n,
                                                              float* out, int32_t* __restrict__ singular)
 {
-    __shared__ float s_prow[2][TG * R];
-    __shared__ float s_pcol[2][TG * R];
+    constexpr int NP = TG * R;                 // padded order
+    __shared__ float s_w[NP * TG];             // W[i][kk] of the current panel
+    __shared__ float s_slab[TG * NP];          // row p_kk before the panel, at every column j: [kk][j]
+    __shared__ float s_prow[2][TG];            // pivot row within the panel
+    __shared__ float s_pcol[2][NP];            // pivot column
     __shared__ float s_cval[2][TG];
     __shared__ int s_cidx[2][TG];
-    __shared__ int s_piv[TG * R];
-    __shared__ int s_step[TG * R];
+    __shared__ int s_piv[NP];
+    __shared__ int s_step[NP];
 
     const int tj = threadIdx.x & (TG - 1);
     const int ti = threadIdx.x >> 5;
@@ -48,7 +235,7 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const
             a[r][c] = (i < n && j < n) ? (float)((i == j ? sh : 0.0) - (double)Bb[(int64_t)i * n + j]) : 0.f;
         }
     }
-    if (threadIdx.x < TG * R) {     // defined permutation entries even when a singular block repeats a row
+    if (threadIdx.x < NP) {         // defined permutation entries even when a singular block repeats a row
         s_piv[threadIdx.x] = 0;
         s_step[threadIdx.x] = 0;
     }
@@ -72,92 +259,8 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const
         s_cidx[0][ti] = bi;
     }
 
-#pragma unroll
-    for (int ck = 0; ck < R; ++ck) {
-        for (int kk = 0; kk < TG; ++kk) {
-            const int k = TG * ck + kk;
-            if (k >= n)
-                break;
-            const int buf = k & 1;
-            __syncthreads();
-
-            // every wavefront reduces the 32 candidates (both 32-lane halves hold the same set)
-            float v = s_cval[buf][threadIdx.x & 31];
-            int p = s_cidx[buf][threadIdx.x & 31];
-#pragma unroll
-            for (int off = 16; off >= 1; off >>= 1) {
-                const float v2 = __shfl_xor(v, off);
-                const int p2 = __shfl_xor(p, off);
-                if (v2 > v || (v2 == v && p2 < p)) {
-                    v = v2;
-                    p = p2;
-                }
-            }
-            if (!(v > 0.f))
-                flag = k + 1;       // no usable pivot (singular block or non-finite input)
-            p = (p < n) ? p : k;    // keeps every index in range when the block is singular
-
-            if (ti == (p & 31)) {
-                const int rp = p >> 5;
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-                    if (r == rp) {
-#pragma unroll
-                        for (int c = 0; c < R; ++c)
-                            s_prow[buf][tj + TG * c] = a[r][c];
-                    }
-                used |= 1u << rp;
-            }
-            if (tj == kk) {
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-                    s_pcol[buf][ti + TG * r] = a[r][ck];
-            }
-            if (threadIdx.x == 0) {
-                s_piv[k] = p;
-                s_step[p] = k;
-            }
-            __syncthreads();
-
-            const float apk = s_prow[buf][k];
-            const float inv = (v > 0.f) ? 1.f / apk : 0.f;
-            float pj[R];
-#pragma unroll
-            for (int c = 0; c < R; ++c)
-                pj[c] = s_prow[buf][tj + TG * c];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = ti + TG * r;
-                const float fo = s_pcol[buf][i] * inv;
-                // pivot row: a_pj (1 - (1 - inv)) = a_pj inv; other rows: a_ij - (a_ik / a_pk) a_pj
-                const float f = (i == p) ? 1.f - inv : fo;
-#pragma unroll
-                for (int c = 0; c < R; ++c)
-                    a[r][c] = fmaf(-f, pj[c], a[r][c]);
-                if (tj == kk)
-                    a[r][ck] = (i == p) ? inv : -fo;
-            }
-
-            // pivot candidates of column k + 1 (unused rows only)
-            const int kn = k + 1;
-            if (kn < n && tj == (kn & 31)) {
-                float best = -1.f;
-                int bi = 0x7fffffff;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int i = ti + TG * r;
-                    const float val = (kk < TG - 1) ? a[r][ck] : (ck + 1 < R ? a[r][(ck + 1) % R] : 0.f);
-                    const float av = fabsf(val);
-                    if (i < n && !((used >> r) & 1u) && av > best) {
-                        best = av;
-                        bi = i;
-                    }
-                }
-                s_cval[buf ^ 1][ti] = best;
-                s_cidx[buf ^ 1][ti] = bi;
-            }
-        }
-    }
+    const InvShared sm{ s_w, s_slab, s_prow, &s_pcol[0][0], s_cval, s_cidx, s_piv, s_step };
+    Panels<R, 0>::run(a, used, flag, n, ti, tj, sm);
     __syncthreads();
 
     float* ob = out + blk * (int64_t)n * n;
