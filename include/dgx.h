@@ -131,8 +131,8 @@ dgx_status dgx_jvp(dgx_ctx* ctx, int apply_minv, int layout, int ptr_kind, const
 /* Volume-diagonal blocks of the linearization at the current state (Euler): for every volume e and
  * unit direction j = v*Ns + n of that volume (neighbours' directions zero), blocks_t[e][j][i] =
  * d(R or M^-1 R)_i / d u_j with i = v'*Ns + n' (the transposed block, row-major; device pointer of
- * n_elem * (nv*Ns)^2 doubles, volumes in list order).  One launch; equals the JVPs of the unit
- * directions restricted to their volume.  Replaces the assembly of the volume-diagonal entries of the
+ * n_elem * (nv*Ns)^2 doubles, volumes in list order).  Computed on the device without probing; equals
+ * the JVPs of the unit directions restricted to their volume.  Replaces the assembly of the volume-diagonal entries of the
  * reference's implicit matrix (compute_rlhs_dg with the lhs, dg/solve_dg.c:138-145) for a block
  * preconditioner. */
 dgx_status dgx_jacobian_blocks(dgx_ctx* ctx, int apply_minv, double* blocks_t);
@@ -145,7 +145,7 @@ dgx_status dgx_jacobian_blocks_f32(dgx_ctx* ctx, int apply_minv, float* blocks_t
  * meaningless).  Gauss-Jordan with partial pivoting, one workgroup per block, launched on `stream`
  * (a hipStream_t; NULL = the null stream) without synchronising.  Replaces the preconditioner set-up of
  * the reference's KSP (constructor_petsc_ksp, solve_implicit.c:562-590: ILU on the assembled matrix),
- * which a matrix-free solver cannot factor; the blocks come from dgx_jvp probes. */
+ * which a matrix-free solver cannot factor; the blocks come from dgx_jacobian_blocks. */
 dgx_status dgx_block_inverse(int64_t n_blocks, int n, const double* blocks, const double* shift, float* inv,
                              int32_t* singular, void* stream);
 /* The same from FP32 blocks; inv may equal blocks (inversion in place). */
