@@ -70,16 +70,21 @@ __device__ __forceinline__ void panel(float (&a)[R][R], unsigned& used, int& fla
             const uint64_t k2 = __shfl_xor(key, off);
             key = k2 > key ? k2 : key;
         }
-        const float v = __uint_as_float((uint32_t)(key >> 32));
-        int p = (int)~(uint32_t)key;
+        // every lane holds the same maximum: read it as a wavefront-uniform (scalar) value, so the row
+        // tests below are scalar compares
+        const uint32_t klo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)key);
+        const uint32_t khi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(key >> 32));
+        const float v = __uint_as_float(khi);
+        int p = (int)~klo;
         if (!(v > 0.f))
             flag = k + 1;       // no usable pivot (singular block or non-finite input)
         p = (p < n) ? p : k;    // keeps every index in range when the block is singular
+        const int rp = p >> 5;
+        const bool prow_lane = ti == (p & 31);
 
-        if (ti == (p & 31)) {
+        if (prow_lane) {
             // the row's value as an OR of masked bit patterns: every index into the register tile stays
             // a constant (a select between tile elements would be turned into a select of addresses)
-            const int rp = p >> 5;
             unsigned bits = 0u;
 #pragma unroll
             for (int r = 0; r < R; ++r)
@@ -99,17 +104,22 @@ __device__ __forceinline__ void panel(float (&a)[R][R], unsigned& used, int& fla
         __syncthreads();
 
         const float apk = s_prow[buf][kk];
-        const float inv = (v > 0.f) ? 1.f / apk : 0.f;
+        // 1 / a_pk: hardware reciprocal (within 1 ulp, exact for powers of two) and one Newton step
+        float inv = __builtin_amdgcn_rcpf(apk);
+        inv = fmaf(fmaf(-apk, inv, 1.f), inv, inv);
+        inv = (v > 0.f) ? inv : 0.f;
         const float pj = s_prow[buf][tj];
+        const bool pcol_lane = tj == kk;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = ti + TG * r;
+            const bool is_p = prow_lane && r == rp;
             const float fo = sm.pcol[buf * NP + i] * inv;
             // pivot row: a_pj (1 - (1 - inv)) = a_pj inv; other rows: a_ij - (a_ik / a_pk) a_pj
-            const float f = (i == p) ? 1.f - inv : fo;
+            const float f = is_p ? 1.f - inv : fo;
             a[r][CK] = fmaf(-f, pj, a[r][CK]);
-            if (tj == kk)
-                a[r][CK] = (i == p) ? inv : -fo;
+            if (pcol_lane)
+                a[r][CK] = is_p ? inv : -fo;
         }
 
         // pivot candidates of column k + 1 within the panel (unused rows only)
