@@ -33,12 +33,15 @@ PREC_FP64, PREC_MIXED = 0, 1
 LAYOUT_REF, LAYOUT_SOA = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
 NODES_GL, NODES_GLL = 0, 1
+KRON_R_FASTEST = 0
+SOLVER_E_SSP_RK_33, SOLVER_E_EULER, SOLVER_E_LS_RK_54 = 0, 1, 2
 
 
 class DgxConfig(ctypes.Structure):
     _fields_ = [("d", ctypes.c_int32), ("p", ctypes.c_int32), ("pde", ctypes.c_int32),
                 ("num_flux", ctypes.c_int32), ("positivity", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("mu", ctypes.c_double), ("Pr", ctypes.c_double)]
+                ("mu", ctypes.c_double), ("Pr", ctypes.c_double), ("solver_type_e", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class DgxMesh(ctypes.Structure):
@@ -58,6 +61,10 @@ class DgxOps1d(ctypes.Structure):
                 ("lp", ctypes.c_double * 8), ("sb", ctypes.c_double * 64), ("bs", ctypes.c_double * 64)]
 
 
+class DgxOperators(ctypes.Structure):
+    _fields_ = [("node_type", ctypes.c_int32), ("kron_order", ctypes.c_int32), ("line", DgxOps1d)]
+
+
 class DgxError(RuntimeError):
     def __init__(self, status, msg=""):
         self.status = status
@@ -68,7 +75,8 @@ class DgxError(RuntimeError):
 SIGNATURES = {
     # dgx.h
     "dgx_create": (ctypes.c_int, [ctypes.POINTER(DgxConfig), ctypes.POINTER(DgxMesh),
-                                  ctypes.POINTER(DgxGeometry), ctypes.POINTER(ctypes.c_void_p)]),
+                                  ctypes.POINTER(DgxGeometry), ctypes.POINTER(DgxOperators),
+                                  ctypes.POINTER(ctypes.c_void_p)]),
     "dgx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "dgx_jacobian_blocks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "dgx_jacobian_blocks_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
@@ -90,6 +98,7 @@ SIGNATURES = {
                                ctypes.c_void_p, c_double_p]),
     "dgx_rk_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, c_double_p]),
     "dgx_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, c_double_p]),
+    "dgx_stages_per_step": (ctypes.c_int, [ctypes.c_void_p]),
     "dgx_sync": (ctypes.c_int, [ctypes.c_void_p, c_int64_p]),
     "dgx_vjp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                ctypes.c_void_p]),

@@ -19,6 +19,7 @@
 struct dgx_ctx {
     int d = 0, p = 0, nv = 0, ns = 0, nf = 0, nface = 0;
     int flux = 0, positivity = 1, precision = 0;
+    int scheme = 0, n_stages = 3;   // time stepper (dgx::RK_*) and its stages per step
     bool visc = false;           // Navier-Stokes (BR2) instead of Euler
     double mu = 0.0, Pr = 1.0, eta = 0.0;
     double* gtr[2] = { nullptr, nullptr };  // NS: [Ep][nface][d*nv][nf] face traces of the volume gradient;
@@ -49,6 +50,7 @@ struct dgx_ctx {
     double* dmax = nullptr;      // [1]
     unsigned long long* bad_elem = nullptr;
     unsigned int* nonfinite = nullptr;
+    unsigned int* gate = nullptr;   // [2] stage gate words, indexed by trace parity (dgx::stage_gated)
     int cur = 0;                 // index of the trace buffer holding the current state's traces
     int state_buf = 0;           // HEX p=3: the current state is in u (0) or in up (1) (RK buffer rotation)
     int rk_next = 0;             // next SSP-RK3 stage dgx_rk_stage accepts
@@ -84,7 +86,8 @@ void free_all(dgx_ctx* c)
     }
     void* ptrs[] = { c->u, c->up, c->rhs, c->stage_buf, c->met, c->imass, c->tr[0], c->tr[1], c->nbr,
                      c->slot_elem, c->elem_slot, c->face_list, c->face_count, c->blockmax, c->dmax,
-                     c->bad_elem, c->nonfinite, c->lin_x, c->lin_tr, c->gtr[0], c->gtr[1], c->lam, c->fnv };
+                     c->bad_elem, c->nonfinite, c->gate, c->lin_x, c->lin_tr, c->gtr[0], c->gtr[1], c->lam,
+                     c->fnv };
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -100,8 +103,17 @@ dgx_status reset_flags(dgx_ctx* c)
 {
     HIP_TRY(c, dgx::launch_fill_u64(c->bad_elem, 1, ~0ull, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->nonfinite, 0, sizeof(unsigned int), c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->gate, 0, 2 * sizeof(unsigned int), c->stream));
     return DGX_OK;
 }
+
+// Low-storage RK 5-stage 4th order: Carpenter & Kennedy 1994, solution 3 of table 1, rational form
+// (the coefficient table of time_step_ls_rk_54, solve_explicit.c:310-321).
+const double kLs54A[5] = { 0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                           -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0 };
+const double kLs54B[5] = { 1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                           17201463215490.0 / 20902069494980.0, 3134564353537.0 / 4481467310338.0,
+                           2277821191437.0 / 14882151754819.0 };
 
 dgx_status check_flags(dgx_ctx* c, int64_t* elem)
 {
@@ -120,15 +132,34 @@ dgx_status check_flags(dgx_ctx* c, int64_t* elem)
     return DGX_OK;
 }
 
-// Launches one stage (stage 0..2) or the residual (stage -1) reading traces tr[tin].
+// Time-stepper fields of the kernel arguments: scheme, LS-RK54 coefficients, the stage gate words
+// (read the one of the trace parity tin, write the other).
+template <class Args>
+void set_rk(const dgx_ctx* c, int stage, int tin, Args& a)
+{
+    a.scheme = c->scheme;
+    if (c->scheme == dgx::RK_LS54 && stage >= 0) {
+        a.rk_a = kLs54A[stage];
+        a.rk_b = kLs54B[stage];
+    }
+    a.gate_in = c->gate + tin;
+    a.gate_out = c->gate + (tin ^ 1);
+}
+
+// Launches one stage (stage 0..n_stages-1) or the residual (stage -1) reading traces tr[tin].
 hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin, int precision)
 {
     if (c->hex3) {
         dgx::Hex3Args a;
         std::memset(&a, 0, sizeof(a));
-        // buffer rotation: stage 0 u -> up, stage 1 up (+u^n in u) -> up, stage 2 up (+u^n) -> u
+        // SSP-RK3 buffer rotation: stage 0 u -> up, stage 1 up (+u^n in u) -> up, stage 2 up (+u^n) -> u;
+        // the other steppers update u in place with up as their RK register
         if (stage < 0) {
             a.u = c->state_buf ? c->up : c->u;
+        } else if (c->scheme != dgx::RK_SSP33) {
+            a.u = c->u;
+            a.u_out = c->u;
+            a.reg = c->up;
         } else if (stage == 0) {
             a.u = c->u;
             a.u_out = c->up;
@@ -154,6 +185,7 @@ hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin,
         a.stage = stage;
         a.apply_minv = apply_minv;
         a.positivity = c->positivity;
+        set_rk(c, stage, tin, a);
         return dgx::launch_hex3_stage(c->flux, precision == DGX_PREC_MIXED, a, c->stream);
     }
     dgx::StageArgs a;
@@ -173,6 +205,7 @@ hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin,
     a.stage = stage;
     a.apply_minv = apply_minv;
     a.positivity = c->positivity;
+    set_rk(c, stage, tin, a);
     if (c->visc) {
         a.gtr = c->fnv ? c->gtr[tin] : c->gtr[0];
         a.gtr_out = c->fnv ? c->gtr[tin ^ 1] : nullptr;
@@ -216,6 +249,24 @@ dgx_status download_state(dgx_ctx* c, int layout, int ptr_kind, const double* sr
     if (ptr_kind == DGX_PTR_HOST)
         HIP_TRY(c, hipMemcpyAsync(dst, c->stage_buf, bytes, hipMemcpyDeviceToHost, c->stream));
     return DGX_OK;
+}
+
+// Entry-by-entry agreement (1e-13 relative to max(1, |entry|)) of a caller's operator set with the built one.
+bool operators_match(const dgx_operators& in, const dgx_ops1d& ref)
+{
+    if (in.node_type != DGX_NODES_GL || in.kron_order != DGX_KRON_R_FASTEST || in.line.p != ref.p ||
+        in.line.n != ref.n)
+        return false;
+    const int n = ref.n;
+    auto near = [](const double* a, const double* b, int k) {
+        for (int i = 0; i < k; ++i)
+            if (!(std::fabs(a[i] - b[i]) <= 1e-13 * std::fmax(1.0, std::fabs(b[i]))))
+                return false;
+        return true;
+    };
+    return near(in.line.x, ref.x, n) && near(in.line.w, ref.w, n) && near(in.line.D, ref.D, n * n) &&
+           near(in.line.lm, ref.lm, n) && near(in.line.lp, ref.lp, n) && near(in.line.sb, ref.sb, n * n) &&
+           near(in.line.bs, ref.bs, n * n);
 }
 
 // Groups of up to 8 elements: 2x2x2 bricks found through the +r/+s/+t neighbours, the remaining
@@ -292,7 +343,8 @@ const char* dgx_last_error(const dgx_ctx* ctx)
     return ctx ? ctx->err.c_str() : "null context";
 }
 
-dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geometry* geom, dgx_ctx** out)
+dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geometry* geom,
+                      const dgx_operators* ops_in, dgx_ctx** out)
 {
     if (!out)
         return DGX_ERR_INVALID_ARG;
@@ -305,11 +357,24 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         return DGX_ERR_INVALID_ARG;
     if (!dgx::supported_order(d, p) || mesh->d != d || (cfg->pde != DGX_PDE_EULER && !visc) ||
         (cfg->num_flux != DGX_FLUX_LAX_FRIEDRICHS && cfg->num_flux != DGX_FLUX_ROE_PIKE) ||
-        (cfg->precision != DGX_PREC_FP64 && cfg->precision != DGX_PREC_MIXED))
+        (cfg->precision != DGX_PREC_FP64 && cfg->precision != DGX_PREC_MIXED) ||
+        (cfg->solver_type_e != DGX_SOLVER_E_SSP_RK_33 && cfg->solver_type_e != DGX_SOLVER_E_EULER &&
+         cfg->solver_type_e != DGX_SOLVER_E_LS_RK_54))
         return DGX_ERR_INVALID_ARG;
     if (mesh->n_elem < 1 || mesh->n_elem >= (1 << 27) || mesh->n_face < 1 || !mesh->face_elem || !mesh->face_lf ||
         !mesh->face_ord || !geom->metrics_vc || !geom->jdet_vc)
         return DGX_ERR_INVALID_ARG;
+
+    // operator set: built for p, or the caller's after checking that it is the GL/GL set of order p (the
+    // kernels' diagonal inverse mass and collocated cubature are exact for it only); its values are applied
+    dgx_ops1d o;
+    if (dgx_ops1d_build(p, &o) != DGX_OK)
+        return DGX_ERR_INVALID_ARG;
+    if (ops_in) {
+        if (!operators_match(*ops_in, o))
+            return DGX_ERR_INVALID_ARG;
+        o = ops_in->line;
+    }
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -352,6 +417,8 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     c->nf = (d == 2) ? n1 : n1 * n1;
     c->nface = nface;
     c->flux = cfg->num_flux;
+    c->scheme = cfg->solver_type_e;
+    c->n_stages = (c->scheme == DGX_SOLVER_E_EULER) ? 1 : (c->scheme == DGX_SOLVER_E_LS_RK_54 ? 5 : 3);
     c->positivity = cfg->positivity ? 1 : 0;
     c->precision = cfg->precision;
     c->E = E;
@@ -418,11 +485,6 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         }
     }
 
-    dgx_ops1d o;
-    if (dgx_ops1d_build(p, &o) != DGX_OK) {
-        delete c;
-        return DGX_ERR_INVALID_ARG;
-    }
     dgx::Ops1D ops;
     std::memset(&ops, 0, sizeof(ops));
     for (int i = 0; i < n1; ++i) {
@@ -529,6 +591,7 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     step(hipMalloc(&c->dmax, sizeof(double)), "hipMalloc(dmax)");
     step(hipMalloc(&c->bad_elem, sizeof(unsigned long long)), "hipMalloc(bad)");
     step(hipMalloc(&c->nonfinite, sizeof(unsigned int)), "hipMalloc(nonfinite)");
+    step(hipMalloc(&c->gate, 2 * sizeof(unsigned int)), "hipMalloc(gate)");
     if (st == DGX_OK) {
         step(hipMemcpy(c->met, met.data(), sizeof(double) * met.size(), hipMemcpyHostToDevice), "copy met");
         step(hipMemcpy(c->imass, imass.data(), sizeof(double) * imass.size(), hipMemcpyHostToDevice), "copy imass");
@@ -667,17 +730,17 @@ dgx_status dgx_rk_stage(dgx_ctx* ctx, int stage, double dt, double* max_rhs)
 {
     if (!ctx)
         return DGX_ERR_INVALID_ARG;
-    if (stage < 0 || stage > 2 || !std::isfinite(dt))
+    if (stage < 0 || stage >= ctx->n_stages || !std::isfinite(dt))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rk_stage: bad arguments");
     if (stage != ctx->rk_next)
-        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rk_stage: stages run in the order 0, 1, 2 (expected stage " +
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rk_stage: stages run in order (expected stage " +
                                                   std::to_string(ctx->rk_next) + ")");
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     HIP_TRY(ctx, launch_any(ctx, stage, dt, 1, ctx->cur, ctx->precision));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     ctx->cur ^= 1;
-    ctx->rk_next = (stage + 1) % 3;
-    if (ctx->hex3)
+    ctx->rk_next = (stage + 1) % ctx->n_stages;
+    if (ctx->hex3 && ctx->scheme == dgx::RK_SSP33)
         ctx->state_buf = (stage == 2) ? 0 : 1;
     dgx_status st = read_max(ctx, max_rhs);
     if (st != DGX_OK)
@@ -699,7 +762,7 @@ static dgx_status build_graph(dgx_ctx* c, int parity, double dt)
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int cur = parity;
     hipError_t le = hipSuccess;
-    for (int s = 0; s < 3 && le == hipSuccess; ++s) {
+    for (int s = 0; s < c->n_stages && le == hipSuccess; ++s) {
         le = launch_any(c, s, dt, 1, cur, c->precision);
         cur ^= 1;
     }
@@ -722,7 +785,7 @@ dgx_status dgx_step(dgx_ctx* ctx, double dt, int nsteps, double* max_rhs)
     if (nsteps < 0 || !std::isfinite(dt))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_step: bad arguments");
     if (ctx->rk_next != 0)
-        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_step: an SSP-RK3 step started by dgx_rk_stage is unfinished");
+        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_step: a step started by dgx_rk_stage is unfinished");
     if (nsteps == 0)
         return DGX_OK;
     if (dt != ctx->graph_dt || !ctx->graph[0] || !ctx->graph[1]) {
@@ -736,7 +799,7 @@ dgx_status dgx_step(dgx_ctx* ctx, double dt, int nsteps, double* max_rhs)
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     for (int s = 0; s < nsteps; ++s) {
         HIP_TRY(ctx, hipGraphLaunch(ctx->graph[ctx->cur], ctx->stream));
-        ctx->cur ^= 1;   // three stages flip the trace parity
+        ctx->cur ^= 1;   // an odd number of stages (3, 1 or 5) flips the trace parity
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     dgx_status st = read_max(ctx, max_rhs);
@@ -747,6 +810,11 @@ dgx_status dgx_step(dgx_ctx* ctx, double dt, int nsteps, double* max_rhs)
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
         ctx->last_ms = ms;
     return check_flags(ctx, nullptr);
+}
+
+int dgx_stages_per_step(const dgx_ctx* ctx)
+{
+    return ctx ? ctx->n_stages : 0;
 }
 
 dgx_status dgx_sync(dgx_ctx* ctx, int64_t* elem)

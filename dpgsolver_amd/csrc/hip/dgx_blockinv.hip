@@ -217,7 +217,7 @@ struct Panels<R, R> {
 // TIN = float allows out == B (in place): every block reads all its entries before any is written.
 template <int R, class TIN>
 __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const double* __restrict__ shift, int n,
-                                                             float* out, int32_t* __restrict__ singular)
+                                                             float* out, int32_t* __restrict__ singular, int64_t base)
 {
     constexpr int NP = TG * R;                 // padded order
     __shared__ float s_w[NP * TG];             // W[i][kk] of the current panel
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const
 
     const int tj = threadIdx.x & (TG - 1);
     const int ti = threadIdx.x >> 5;
-    const int64_t blk = blockIdx.x;
+    const int64_t blk = base + blockIdx.x;
     const TIN* Bb = B + blk * (int64_t)n * n;
     const double sh = shift[blk];
 
@@ -296,10 +296,10 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const
 // consecutive floats of one column; x_b is staged in LDS and read as a broadcast.
 __global__ __launch_bounds__(256) void block_apply_kernel(const float* __restrict__ col, int n,
                                                           const double* __restrict__ x, double* __restrict__ y,
-                                                          const int32_t* __restrict__ sel)
+                                                          const int32_t* __restrict__ sel, int64_t base)
 {
     __shared__ float sx[320];
-    const int64_t blk = sel ? sel[blockIdx.x] : blockIdx.x;
+    const int64_t blk = sel ? sel[base + blockIdx.x] : base + blockIdx.x;
     const float* A = col + blk * (int64_t)n * n;
     for (int j = threadIdx.x; j < n; j += 256)
         sx[j] = (float)x[blk * n + j];
@@ -318,13 +318,36 @@ __global__ __launch_bounds__(256) void block_apply_kernel(const float* __restric
     }
 }
 
+// A dispatch holds at most 2^32 - 1 work-items: blocks go out in launches of at most 2^20 workgroups
+// (2^30 work-items at 1024 threads), each with the index of its first block.
+constexpr int64_t kMaxGroups = int64_t(1) << 20;
+
 template <int R, class TIN>
 hipError_t launch(int64_t nb, int n, const TIN* B, const double* shift, float* out, int32_t* singular,
                   hipStream_t st)
 {
-    hipLaunchKernelGGL((block_inverse_kernel<R, TIN>), dim3((unsigned)nb), dim3(1024), 0, st, B, shift, n, out,
-                       singular);
-    return hipGetLastError();
+    for (int64_t base = 0; base < nb; base += kMaxGroups) {
+        const int64_t cnt = (nb - base < kMaxGroups) ? nb - base : kMaxGroups;
+        hipLaunchKernelGGL((block_inverse_kernel<R, TIN>), dim3((unsigned)cnt), dim3(1024), 0, st, B, shift, n, out,
+                           singular, base);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_apply(int64_t nb, int n, const float* col, const double* x, double* y, const int32_t* sel,
+                        hipStream_t st)
+{
+    for (int64_t base = 0; base < nb; base += kMaxGroups) {
+        const int64_t cnt = (nb - base < kMaxGroups) ? nb - base : kMaxGroups;
+        hipLaunchKernelGGL(block_apply_kernel, dim3((unsigned)cnt), dim3(256), 0, st, col, n, x, y, sel, base);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 template <class TIN>
@@ -379,9 +402,9 @@ extern "C" dgx_status dgx_block_apply(int64_t n_blocks, int n, const float* bloc
         return DGX_OK;
     if (!blocks_colmajor || !x || !y)
         return DGX_ERR_INVALID_ARG;
-    hipLaunchKernelGGL(dgx::block_apply_kernel, dim3((unsigned)n_blocks), dim3(256), 0, (hipStream_t)stream,
-                       blocks_colmajor, n, x, y, (const int32_t*)nullptr);
-    return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+    return dgx::launch_apply(n_blocks, n, blocks_colmajor, x, y, nullptr, (hipStream_t)stream) == hipSuccess
+               ? DGX_OK
+               : DGX_ERR_HIP;
 }
 
 extern "C" dgx_status dgx_block_apply_sel(int64_t n_sel, const int32_t* sel, int n, const float* blocks_colmajor,
@@ -393,7 +416,6 @@ extern "C" dgx_status dgx_block_apply_sel(int64_t n_sel, const int32_t* sel, int
         return DGX_OK;
     if (!sel || !blocks_colmajor || !x || !y)
         return DGX_ERR_INVALID_ARG;
-    hipLaunchKernelGGL(dgx::block_apply_kernel, dim3((unsigned)n_sel), dim3(256), 0, (hipStream_t)stream,
-                       blocks_colmajor, n, x, y, sel);
-    return hipGetLastError() == hipSuccess ? DGX_OK : DGX_ERR_HIP;
+    return dgx::launch_apply(n_sel, n, blocks_colmajor, x, y, sel, (hipStream_t)stream) == hipSuccess ? DGX_OK
+                                                                                                     : DGX_ERR_HIP;
 }

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(BLOCK) void geometry_kernel(GeoOps gk, const double
                                                          const int8_t* __restrict__ face_lf,
                                                          double* __restrict__ met, double* __restrict__ jdet,
                                                          double* __restrict__ nrm, double* __restrict__ jdf,
-                                                         int* __restrict__ bad)
+                                                         int* __restrict__ bad, int64_t base)
 {
     // geometry-node fields, then the two intermediate grids of the sum-factorised interpolation: the
     // pass-s output (18 fields) reuses sM and sX, which are no longer read once pass r is complete
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(BLOCK) void geometry_kernel(GeoOps gk, const double
     // pass-s output slot k: sM for k < 9, sX for 9 <= k < 12, the last 6 rows for 12 <= k < 18
 #define SW(k) sbuf[(k) < 9 ? 18 + (k) : ((k) < 12 ? (k) - 9 : 15 + (k))]
     __shared__ GeoOps g;                // the 1-D operators, read with per-thread indices
-    const int64_t e = blockIdx.x;
+    const int64_t e = base + blockIdx.x;
     const int t = threadIdx.x;
     if (t == 0)
         g = gk;
@@ -252,12 +252,17 @@ void launch_geometry(bool small, const GeoOps& g, const double* x, int64_t E, co
                      const int64_t* list, const int8_t* lf, double* om, double* oj, double* on, double* of, int* bad,
                      hipStream_t st)
 {
-    if (small)
-        hipLaunchKernelGGL((geometry_kernel<D, 64>), dim3((unsigned)E), dim3(BLOCK), 0, st, g, x, E, wf, first, list,
-                           lf, om, oj, on, of, bad);
-    else
-        hipLaunchKernelGGL((geometry_kernel<D, VMAX>), dim3((unsigned)E), dim3(BLOCK), 0, st, g, x, E, wf, first,
-                           list, lf, om, oj, on, of, bad);
+    // at most 2^20 workgroups per dispatch (a dispatch holds at most 2^32 - 1 work-items)
+    const int64_t kMaxGroups = int64_t(1) << 20;
+    for (int64_t base = 0; base < E; base += kMaxGroups) {
+        const unsigned cnt = (unsigned)((E - base < kMaxGroups) ? E - base : kMaxGroups);
+        if (small)
+            hipLaunchKernelGGL((geometry_kernel<D, 64>), dim3(cnt), dim3(BLOCK), 0, st, g, x, E, wf, first, list, lf,
+                               om, oj, on, of, bad, base);
+        else
+            hipLaunchKernelGGL((geometry_kernel<D, VMAX>), dim3(cnt), dim3(BLOCK), 0, st, g, x, E, wf, first, list,
+                               lf, om, oj, on, of, bad, base);
+    }
 }
 
 struct DevBuf {

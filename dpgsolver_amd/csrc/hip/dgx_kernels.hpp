@@ -269,6 +269,43 @@ __device__ inline void numflux(const T* wl, const T* wr, const double* n, T* nf)
         numflux_roe<D>(wl, wr, n, nf);
 }
 
+// Explicit time steppers (the values of dgx.h DGX_SOLVER_E_*).
+enum { RK_SSP33 = 0, RK_EULER = 1, RK_LS54 = 2 };
+
+// Stage combination of one state value s with its residual r (M^-1 applied) for stage `stage` of the
+// scheme; reg is the value's RK register (sol_coef_p), read and written as the scheme needs.
+//   SSP-RK3  time_step_ssp_rk_33 (solve_explicit.c:271-293): the register keeps u^n from stage 0 on
+//   Euler    time_step_euler (solve_explicit.c:240-244)
+//   LS-RK54  time_step_ls_rk_54 (solve_explicit.c:338-342): reg = a reg + dt r, s += b reg; the first
+//            stage's a is 0, so the register's old content is not read
+__device__ inline double rk_combine(int scheme, int stage, double rk_a, double rk_b, double dt, double s, double r,
+                                    double* reg)
+{
+    if (scheme == RK_EULER)
+        return s + dt * r;
+    if (scheme == RK_LS54) {
+        const double sp = (stage == 0) ? dt * r : rk_a * (*reg) + dt * r;
+        *reg = sp;
+        return s + rk_b * sp;
+    }
+    if (stage == 0) {
+        *reg = s;
+        return s + dt * r;
+    }
+    if (stage == 1)
+        return (1.0 / 4.0) * (3.0 * (*reg) + s + dt * r);
+    return (1.0 / 3.0) * ((*reg) + 2.0 * s + 2.0 * dt * r);
+}
+
+// Stage gate: a stage kernel reads gate_in, the word only earlier stages write, and writes gate_out
+// (the other word) when one of its elements fails the positivity check or when it is itself gated.
+// gate_in does not change while the kernel runs, so every wavefront of the launch takes the same
+// decision without a barrier; once set, both words stay set until the host clears them.
+__device__ inline bool stage_gated(int stage, const unsigned int* gate_in)
+{
+    return stage >= 0 && *gate_in != 0u;
+}
+
 struct StageArgs {
     double* u;              // [NV][E][NS] state, updated in place
     double* up;             // [NV][E][NS] RK register (sol_coef_p)
@@ -282,9 +319,13 @@ struct StageArgs {
     unsigned long long* bad_elem;  // first element that failed the positivity check
     int64_t n_elem;
     double dt;
-    int stage;              // 0,1,2: SSP-RK3 stage; -1: residual only
+    int stage;              // stage of the time stepper; -1: residual only
     int apply_minv;         // residual mode: 1 = M^-1 applied
     int positivity;
+    int scheme;             // RK_*
+    double rk_a, rk_b;      // LS-RK54 coefficients of this stage
+    const unsigned int* gate_in;   // stage gate (stage_gated)
+    unsigned int* gate_out;
     // Navier-Stokes only
     const double* gtr;      // [E][NFACE][D*NV][NF] face traces of the volume gradient of the current state
                             // (HEX p=4: [E][NFACE][NV][NF] traces of n.f2(w, grad w), outward unit n)
@@ -404,11 +445,16 @@ struct Hex3Args {
     double* rhs_out;            // [NV][Ep][64] residual mode output
     double* blockmax;           // [Ep] max |rhs| per element
     unsigned long long* bad_elem;
+    double* reg;                // [NV][Ep][64] RK register of the Euler / LS-RK54 steppers (state in place)
     int64_t n_slots;
     double dt;
     int stage;
     int apply_minv;
     int positivity;
+    int scheme;                 // RK_*: SSP-RK3 rotates u / up / u_out as above, the others update u in place
+    double rk_a, rk_b;          // LS-RK54 coefficients of this stage
+    const unsigned int* gate_in;   // stage gate (stage_gated)
+    unsigned int* gate_out;
 };
 
 // Linearized residual (J w or J^T v) at the state u; arrays in the internal [v][slot][node] order.

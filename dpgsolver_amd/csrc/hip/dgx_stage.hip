@@ -185,24 +185,14 @@ __device__ inline void stage_tail(const StageArgs& a, int64_t e, int n, int el, 
         return;
     }
 
-    // ---- SSP-RK3 stage combination (solve_explicit.c:271-293) -------------------------------------
+    // ---- stage combination of the time stepper (solve_explicit.c:226-350) ----------------------------
     double un[NV];
     if (valid) {
         const double dt = a.dt;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const int64_t gi = (v * E + e) * NS + n;
-            const double s = uu[v], r = rhs[v];
-            if (a.stage == 0) {
-                a.up[gi] = s;
-                un[v] = s + dt * r;
-            } else if (a.stage == 1) {
-                const double sp = a.up[gi];
-                un[v] = (1.0 / 4.0) * (3.0 * sp + s + dt * r);
-            } else {
-                const double sp = a.up[gi];
-                un[v] = (1.0 / 3.0) * (sp + 2.0 * s + 2.0 * dt * r);
-            }
+            un[v] = rk_combine(a.scheme, a.stage, a.rk_a, a.rk_b, dt, uu[v], rhs[v], a.up + gi);
         }
     }
     __syncthreads();   // sg / snnf reads are complete before su is overwritten
@@ -289,8 +279,10 @@ __device__ inline void stage_tail(const StageArgs& a, int64_t e, int n, int el, 
                 }
             }
             (void)changed;
-            if (valid && failed && n == 0)
+            if (valid && failed && n == 0) {
                 atomicMin(a.bad_elem, (unsigned long long)e);
+                *a.gate_out = 1u;
+            }
             if (valid && !failed) {
                 // primitive -> conservative (solution_euler_T.c:152-167), then Bezier -> Lagrange (ccBS0)
                 double V2 = 0.0;
@@ -365,6 +357,11 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs a)
     const int64_t E = a.n_elem;
     const int64_t e = (int64_t)blockIdx.x * EPB + el;
     const bool valid = in_tile && e < E;
+    if (stage_gated(a.stage, a.gate_in)) {   // an earlier stage failed: the state stays as it left it
+        if (tid == 0)
+            *a.gate_out = 1u;
+        return;
+    }
 
     double* su = s_reg + (in_tile ? el : 0) * REGION;   // [NV][NS]
     double* sm = su + NV * NS;                          // [DD][NS]

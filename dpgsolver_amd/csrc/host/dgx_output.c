@@ -55,10 +55,9 @@ static void tp_apply_o(int d, int n_out, int n_in, const double* M, const double
 
 typedef struct vnode {
     int idx;
+    int d;          /* coordinates compared (the mesh dimension); kept per node so the sort is reentrant */
     double x[3];
 } vnode;
-
-static int g_dim_cmp;
 
 /* equal_spec_rel_T with den = x0 (math_functions_T.c:34-48) */
 static int equal_rel(double x0, double x1, double tol)
@@ -71,7 +70,7 @@ static int cmp_vnode(const void* pa, const void* pb)
 {
     const vnode* a = (const vnode*)pa;
     const vnode* b = (const vnode*)pb;
-    for (int k = g_dim_cmp - 1; k >= 0; --k) {
+    for (int k = a->d - 1; k >= 0; --k) {
         if (equal_rel(a->x[k], b->x[k], NODE_TOL))
             continue;
         return a->x[k] < b->x[k] ? -1 : 1;
@@ -107,11 +106,11 @@ dgx_status dgx_restart_write(const char* path, const dgx_mesh* mesh, int p, cons
             const int a = (ng - 1) * ((v & 1) + ng * (((v >> 1) & 1) + ng * ((v >> 2) & 1)));
             vnode* nd = &nodes[e * nve + v];
             nd->idx = (int)(e * nve + v);
+            nd->d = d;
             for (int k = 0; k < 3; ++k)
                 nd->x[k] = k < d ? mesh->xyz_g[(e * d + k) * Ng + a] : 0.0;
         }
     /* row_sort_DIM_Matrix_d + make_unique_row_Multiarray_d (restart_writers.c:186-196) */
-    g_dim_cmp = d;
     qsort(nodes, (size_t)NV, sizeof(vnode), cmp_vnode);
     for (int64_t i = 0; i < NV; ++i)
         sorted_pos[nodes[i].idx] = (int)i;

@@ -372,15 +372,20 @@ def max_wave_speed(u, sb, d):
 def solve_implicit(solver, mesh, p, cfl_initial=1.0, exit_tol=1e-10, exit_ratio=0.0, max_steps=50,
                    rtol=1e-4, restart=60, max_iter=600, precondition=True, verbose=False, device="cuda",
                    precond=None, sweep="jacobi"):
-    """Pseudo-transient Newton-Krylov from the solver's current state until max |M^-1 R| < exit_tol or
-    max_rhs / max_rhs_0 < exit_ratio (check_exit, solve_implicit.c:258-284).
+    """Pseudo-transient Newton-Krylov from the solver's current state, as the reference's implicit loop
+    (solve_implicit.c:95-108, implicit_step :232-256, check_exit :258-284): every step computes max_rhs,
+    the maximum of |R| over the volumes before M^-1 (compute_rlhs_dg returns compute_max_rhs_dg_like of
+    the un-scaled residual, dg/solve_dg.c:138-145), ramps the CFL number with it
+    (cfl = cfl_initial * max(1, max_rhs_0 / max_rhs), compute_CFL_ramping :223-257), solves and applies
+    the update, and only then stops when max_rhs < exit_tol or max_rhs / max_rhs_0 < exit_ratio (so the
+    update of the step that passes the test is applied), or after max_steps steps.
 
     Defaults follow the reference's GMRES options (restart 60, rtol 1e-4).  Returns (final state
-    [E][nv][Ns] as a torch tensor on `device`, history) with one entry per step: (max_rhs, cfl, GMRES
-    iterations, GMRES relative residual); the last entry holds the final max_rhs.  The solver is left
-    at the final state.  `precond`: a BlockJacobi (or an object with its update/apply) to use instead of
-    a new one (`sweep`: "jacobi" or "gauss-seidel", the latter on the GPU only); precondition=False runs
-    plain GMRES."""
+    [E][nv][Ns] as a torch tensor on `device`, history): one entry (max_rhs, cfl, GMRES iterations, GMRES
+    relative residual) per step, max_rhs being the one of the state the step started from, and a last
+    entry (max_rhs of the final state, 0, 0, 0).  The solver is left at the final state.  `precond`: a
+    BlockJacobi (or an object with its update/apply) to use instead of a new one (`sweep`: "jacobi" or
+    "gauss-seidel", the latter on the GPU only); precondition=False runs plain GMRES."""
     import torch
     from .problem import ops1d
     dev = torch.device(device)
@@ -396,17 +401,15 @@ def solve_implicit(solver, mesh, p, cfl_initial=1.0, exit_tol=1e-10, exit_ratio=
     r = torch.empty_like(u)
     hist = []
     max_rhs0 = None
-    for step in range(max_steps + 1):
+    for step in range(max_steps):
         sync()
         solver.set_state(u)
-        _, mx = solver.compute_rhs(apply_minv=True, out=r)
+        mx = solver.max_rhs(apply_minv=False)
         if not math.isfinite(mx):
             raise FloatingPointError(f"non-finite residual at implicit step {step}")
         if max_rhs0 is None:
             max_rhs0 = mx
-        if mx < exit_tol or mx < exit_ratio * max_rhs0 or step == max_steps:
-            hist.append((mx, 0.0, 0, 0.0))
-            break
+        solver.compute_rhs(apply_minv=True, out=r)
         cfl = cfl_initial * max(1.0, max_rhs0 / mx)
         inv_dt = max_wave_speed(u, sb, mesh.d) / (cfl * h)
         bad = torch.nonzero(~torch.isfinite(inv_dt))
@@ -421,8 +424,11 @@ def solve_implicit(solver, mesh, p, cfl_initial=1.0, exit_tol=1e-10, exit_ratio=
         if verbose:
             print(f"implicit step {step}: max_rhs {mx:.3e} cfl {cfl:.3g} gmres {its} ({rel:.1e})", flush=True)
         u = u + du
+        if mx < exit_tol or mx < exit_ratio * max_rhs0:
+            break
     sync()
     solver.set_state(u)
+    hist.append((solver.max_rhs(apply_minv=False), 0.0, 0, 0.0))
     return u, hist
 
 

@@ -18,6 +18,9 @@ from . import _ffi
 NUM_FLUX = {"lax_friedrichs": _ffi.FLUX_LAX_FRIEDRICHS, "lf": _ffi.FLUX_LAX_FRIEDRICHS,
             "roe_pike": _ffi.FLUX_ROE_PIKE, "roe": _ffi.FLUX_ROE_PIKE}
 PRECISION = {"fp64": _ffi.PREC_FP64, "mixed": _ffi.PREC_MIXED}
+# test_case->solver_type_e (solve_explicit.c:150-170)
+SOLVER_TYPE_E = {"ssp_rk_33": _ffi.SOLVER_E_SSP_RK_33, "euler": _ffi.SOLVER_E_EULER,
+                 "ls_rk_54": _ffi.SOLVER_E_LS_RK_54}
 
 
 class Solver:
@@ -26,10 +29,12 @@ class Solver:
     mesh, geo: dpgsolver_amd.problem.Mesh / Geometry; p: solution order; num_flux: "lf" | "roe";
     positivity: apply enforce_positivity_highorder after every stage (the reference does for Euler and
     Navier-Stokes, test_case.c:63-79); viscous: None (Euler) or (mu, Pr) for Navier-Stokes with the
-    BR2-stable viscous flux.
+    BR2-stable viscous flux; time_stepper: "ssp_rk_33" | "euler" | "ls_rk_54" (solver_type_e);
+    operators: None (built for p) or a dpgsolver_amd._ffi.DgxOperators holding the GL/GL 1-D factors.
     """
 
-    def __init__(self, mesh, geo, p, num_flux="roe", positivity=True, precision="fp64", viscous=None):
+    def __init__(self, mesh, geo, p, num_flux="roe", positivity=True, precision="fp64", viscous=None,
+                 time_stepper="ssp_rk_33", operators=None):
         self._lib = _ffi.load()
         self.d, self.p = mesh.d, p
         self.nv = mesh.d + 2
@@ -43,14 +48,17 @@ class Solver:
         if viscous is not None:
             cfg.pde = _ffi.PDE_NAVIER_STOKES
             cfg.mu, cfg.Pr = float(viscous[0]), float(viscous[1])
+        cfg.solver_type_e = SOLVER_TYPE_E[time_stepper] if isinstance(time_stepper, str) else int(time_stepper)
         self._mesh_c = mesh.c_struct()
         self._geo_c = geo.c_struct()
         self._keep = (mesh, geo)
         ctx = ctypes.c_void_p()
-        st = self._lib.dgx_create(ctypes.byref(cfg), ctypes.byref(self._mesh_c), ctypes.byref(self._geo_c),
+        ops = ctypes.byref(operators) if operators is not None else None
+        st = self._lib.dgx_create(ctypes.byref(cfg), ctypes.byref(self._mesh_c), ctypes.byref(self._geo_c), ops,
                                   ctypes.byref(ctx))
         _ffi.check(st)
         self._ctx = ctx
+        self.n_stages = self._lib.dgx_stages_per_step(ctx)
 
     # -- lifetime ------------------------------------------------------------------------------------
     def close(self):
@@ -98,13 +106,21 @@ class Solver:
                                       kind, ptr, ctypes.byref(mx)))
         return out, mx.value
 
+    def max_rhs(self, apply_minv=True):
+        """max over volumes of ||rhs||_inf of the current state (compute_max_rhs_dg_like), without
+        downloading the residual; apply_minv=False: of the residual before M^-1."""
+        mx = ctypes.c_double(0.0)
+        self._check(self._lib.dgx_rhs(self._ctx, _ffi.PREC_FP64, int(bool(apply_minv)), _ffi.LAYOUT_REF,
+                                      _ffi.PTR_HOST, None, ctypes.byref(mx)))
+        return mx.value
+
     def rk_stage(self, stage, dt):
         mx = ctypes.c_double(0.0)
         self._check(self._lib.dgx_rk_stage(self._ctx, int(stage), float(dt), ctypes.byref(mx)))
         return mx.value
 
     def step(self, dt, nsteps=1):
-        """nsteps SSP-RK3 steps as hipGraph launches; returns max_rhs of the last stage."""
+        """nsteps steps of the time stepper as hipGraph launches; returns max_rhs of the last stage."""
         mx = ctypes.c_double(0.0)
         self._check(self._lib.dgx_step(self._ctx, float(dt), int(nsteps), ctypes.byref(mx)))
         return mx.value

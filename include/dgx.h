@@ -40,6 +40,14 @@ enum { DGX_FLUX_LAX_FRIEDRICHS = 0, DGX_FLUX_ROE_PIKE = 1 };   /* numerical_flux
  * dgx_rhs return DGX_ERR_INVALID_ARG elsewhere).  No reference counterpart: the reference is FP64. */
 enum { DGX_PREC_FP64 = 0, DGX_PREC_MIXED = 1 };
 enum { DGX_LAYOUT_REF = 0, DGX_LAYOUT_SOA = 1 };               /* SOA = device layout u[v][e][n] */
+/* Explicit time stepper, the reference's test_case->solver_type_e (solve_explicit.c:150-170):
+ * SSP-RK3 (3 stages, time_step_ssp_rk_33 :252-301), forward Euler (1 stage, time_step_euler :226-250),
+ * low-storage RK 5-stage 4th order (5 stages, time_step_ls_rk_54 :303-350). */
+enum { DGX_SOLVER_E_SSP_RK_33 = 0, DGX_SOLVER_E_EULER = 1, DGX_SOLVER_E_LS_RK_54 = 2 };
+/* Node families of the 1-D reference line (src/element/nodes.c:70-130: GL = "GLe", GLL = "GLoL"). */
+enum { DGX_NODES_GL = 0, DGX_NODES_GLL = 1 };
+/* Ordering of tensor-product nodes: node n = i + (p+1)*(j + (p+1)*k), r fastest (element_operators_tp.c:90-241). */
+enum { DGX_KRON_R_FASTEST = 0 };
 enum { DGX_PTR_HOST = 0, DGX_PTR_DEVICE = 1 };
 
 typedef struct dgx_config {
@@ -55,6 +63,8 @@ typedef struct dgx_config {
      * per element (compute_grad_coef_dg_T.c, compute_face_rlhs_dg_T.c:276-296) */
     double mu;
     double Pr;
+    int32_t solver_type_e;   /* DGX_SOLVER_E_*: stages of dgx_rk_stage / dgx_step (0 = SSP-RK3) */
+    int32_t reserved;        /* 0 */
 } dgx_config;
 
 typedef struct dgx_mesh {
@@ -75,14 +85,45 @@ typedef struct dgx_geometry {
     const double* jdet_fc;       /* [n_face][Nf]        (jacobian_det_fc) */
 } dgx_geometry;
 
+/* The 1-D operator set of a GL solution/cubature order p (collocated GL/GL):
+ *   x[n], w[n]                       GL nodes and weights, n = p+1
+ *   D[n*n]   D[i*n + a] = l_a'(x_i)  derivative of the solution basis at the cubature nodes
+ *   lm[n], lp[n]                     l_a(-1), l_a(+1)  (face trace rows; cv0_vs_fc for a LINE)
+ *   sb[n*n]  ccSB0 (nodal -> Bezier coefficients), row-major
+ *   bs[n*n]  ccBS0 (Bezier -> nodal), row-major
+ * Reference: the LINE sub-operators that element_operators_tp.c:90-241 Kronecker-multiplies into the
+ * tensor-product operators cv0_vs_vc, cv1_vt_vc, cv0_vs_fc, w_vc, ccSB0_vs_vs, ccBS0_vs_vs. */
+typedef struct dgx_ops1d {
+    int32_t p;
+    int32_t n;
+    double x[8], w[8];
+    double D[64];
+    double lm[8], lp[8];
+    double sb[64], bs[64];
+} dgx_ops1d;
+
+/* The operator set the kernels apply by sum factorization: the 1-D factors and the tag of the
+ * Kronecker ordering that builds the element operators from them (the reference's Solver_Element
+ * operators, element_solver.h:31-74, as their LINE factors). */
+typedef struct dgx_operators {
+    int32_t node_type;    /* DGX_NODES_GL (solution nodes = cubature nodes; GLL is not supported) */
+    int32_t kron_order;   /* DGX_KRON_R_FASTEST */
+    dgx_ops1d line;       /* 1-D factors of order cfg->p */
+} dgx_operators;
+
 typedef struct dgx_ctx dgx_ctx;
 
 /* Create a context on the current HIP device: validates the inputs, copies mesh connectivity and
- * geometry to device memory in the SoA layouts the kernels read, builds the 1-D operators for p.
+ * geometry to device memory in the SoA layouts the kernels read, and loads the operator set.
+ * ops: the 1-D factors (nullable: then built for cfg->p by dgx_ops1d_build).  They must be the GL/GL
+ * set of order cfg->p in r-fastest order, entry by entry within 1e-13 (relative to max(1, |entry|)) of
+ * dgx_ops1d_build's, else DGX_ERR_INVALID_ARG: the kernels' diagonal inverse mass and collocated
+ * cubature are exact for that set only.  The values passed are the ones applied.
  * Replaces the explicit-path part of constructor_derived_computational_elements(IL_SOLVER_DG)
- * (solve_explicit.c:88-89 -> dg/volume_solver_dg_T.c:49-78: sol_coef_p, m_inv) and
- * set_up_solver_geometry's consumers. */
-dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geometry* geom, dgx_ctx** out);
+ * (solve_explicit.c:88-89 -> dg/volume_solver_dg_T.c:49-78: sol_coef_p, m_inv), the operator set of
+ * constructor_derived_Solver_Element (element_solver.c:179-300) and set_up_solver_geometry's consumers. */
+dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geometry* geom,
+                      const dgx_operators* ops, dgx_ctx** out);
 dgx_status dgx_destroy(dgx_ctx* ctx);
 const char* dgx_last_error(const dgx_ctx* ctx);
 const char* dgx_status_string(dgx_status s);
@@ -106,14 +147,23 @@ dgx_status dgx_get_state(dgx_ctx* ctx, int layout, int ptr_kind, double* u);
 dgx_status dgx_rhs(dgx_ctx* ctx, int precision, int apply_minv, int layout, int ptr_kind, double* rhs,
                    double* max_rhs);
 
-/* One SSP-RK3 stage (stage 0,1,2) of time_step_ssp_rk_33 (solve_explicit.c:252-301): residual,
- * stage combination, then enforce_positivity_highorder (solve.c:187-233) when enabled.
- * max_rhs (nullable) as for dgx_rhs, of this stage's residual.  Stages run in the order 0, 1, 2, as
- * time_step_ssp_rk_33 runs them (another order returns DGX_ERR_INVALID_ARG; dgx_set_state restarts). */
+/* One stage (0 .. n-1; n = 3, 1 or 5 for cfg->solver_type_e) of time_step_ssp_rk_33 / time_step_euler /
+ * time_step_ls_rk_54 (solve_explicit.c:226-350): residual, stage combination, then
+ * enforce_positivity_highorder (solve.c:187-233) when enabled.  max_rhs (nullable) as for dgx_rhs, of
+ * this stage's residual.  Stages run in the order 0, 1, .., n-1, as the reference's step functions run
+ * them (another order returns DGX_ERR_INVALID_ARG; dgx_set_state restarts).
+ * Positivity failure: the failing stage completes on every element (the failing element keeps its
+ * un-limited stage value) and DGX_ERR_POSITIVITY is returned; every later stage is a no-op that leaves
+ * the state as the failing stage wrote it, until dgx_set_state (the reference aborts, solve.c:204-208). */
 dgx_status dgx_rk_stage(dgx_ctx* ctx, int stage, double dt, double* max_rhs);
 
-/* nsteps full SSP-RK3 steps with fixed dt, launched as a captured hipGraph.  max_rhs (nullable) of
- * the last stage.  Replaces the stepping loop of solve_explicit (solve_explicit.c:78-118). */
+/* dgx_stages_per_step: n of dgx_rk_stage (3, 1 or 5). */
+int dgx_stages_per_step(const dgx_ctx* ctx);
+
+/* nsteps full steps of the time stepper with fixed dt, launched as a captured hipGraph.  max_rhs
+ * (nullable) of the last stage.  Replaces the stepping loop of solve_explicit (solve_explicit.c:78-118).
+ * A positivity failure stops the computation at the failing stage (the stages after it do not change
+ * the state; see dgx_rk_stage) and returns DGX_ERR_POSITIVITY. */
 dgx_status dgx_step(dgx_ctx* ctx, double dt, int nsteps, double* max_rhs);
 
 /* Wait for all work on the context's stream and report a sticky device-side error

@@ -22,33 +22,15 @@
 extern "C" {
 #endif
 
-/* Node families of the 1-D reference line (src/element/nodes.c:70-130: GL = "GLe", GLL = "GLoL"). */
-enum { DGX_NODES_GL = 0, DGX_NODES_GLL = 1 };
-
-/* 1-D nodes and weights on [-1,1], n = p+1 points (GLL requires p >= 1).  x[n], w[n]. */
+/* 1-D nodes and weights on [-1,1], n = p+1 points (GLL requires p >= 1).  x[n], w[n].
+ * family: DGX_NODES_GL or DGX_NODES_GLL (dgx.h). */
 dgx_status dgx_nodes_1d(int family, int p, double* x, double* w);
 
 /* 1-D Lagrange basis on nodes xs[ns] evaluated (der = 0) or differentiated (der = 1) at points y[ny]:
  * out[i*ns + j] = l_j^(der)(y_i)  (row-major ny x ns). */
 dgx_status dgx_lagrange_1d(int ns, const double* xs, int ny, const double* y, int der, double* out);
 
-/* The 1-D operator set the kernels use for a GL solution/cubature order p (collocated GL/GL):
- *   x[n], w[n]                       GL nodes and weights, n = p+1
- *   D[n*n]   D[i*n + a] = l_a'(x_i)  derivative of the solution basis at the cubature nodes
- *   lm[n], lp[n]                     l_a(-1), l_a(+1)  (face trace rows; cv0_vs_fc for a LINE)
- *   sb[n*n]  ccSB0 (nodal -> Bezier coefficients), row-major
- *   bs[n*n]  ccBS0 (Bezier -> nodal), row-major
- * Reference: the LINE sub-operators that element_operators_tp.c:90-241 Kronecker-multiplies
- * (cv0_vs_vc, cv1_vt_vc, cv0_vs_fc, w_vc, ccSB0_vs_vs, ccBS0_vs_vs). */
-typedef struct dgx_ops1d {
-    int32_t p;
-    int32_t n;
-    double x[8], w[8];
-    double D[64];
-    double lm[8], lp[8];
-    double sb[64], bs[64];
-} dgx_ops1d;
-
+/* The 1-D operator set (dgx_ops1d, dgx.h) of the GL solution/cubature order p. */
 dgx_status dgx_ops1d_build(int p, dgx_ops1d* ops);
 
 /* Number of elements / faces / geometry nodes of a structured periodic mesh with n[0..d-1] cells. */

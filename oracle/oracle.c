@@ -44,6 +44,8 @@ struct orc_ctx {
     double eta;                 /* BR2 penalty override (> 0), else 1.01 x NFMAX */
     /* threads of the residual and stage loops (1: the reference's sequential order) */
     int nthreads;
+    /* explicit time stepper of orc_rk_stage: 0 SSP-RK3, 1 forward Euler, 2 low-storage RK 5-stage 4th order */
+    int time_stepper;
     int64_t* ef_face;           /* [E][2d] face of each local face, built on first threaded use */
     int8_t* ef_side;            /* [E][2d] 0: the volume is the face's left volume, 1: its right */
 };
@@ -873,6 +875,18 @@ int64_t orc_positivity(orc_ctx* c, double* u)
 
 /* SSP-RK3 combination (time_step_ssp_rk_33, solve_explicit.c:271-293) and positivity of volume e; 0 when
  * the volume fails the positivity check. */
+/* Low-storage RK 5-stage 4th order: Carpenter & Kennedy 1994, solution 3 of table 1, rational form
+ * (the coefficient table of time_step_ls_rk_54, solve_explicit.c:310-321). */
+static const double k_ls54_a[5] = { 0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                                    -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0 };
+static const double k_ls54_b[5] = { 1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                                    17201463215490.0 / 20902069494980.0, 3134564353537.0 / 4481467310338.0,
+                                    2277821191437.0 / 14882151754819.0 };
+
+/* Stage update of one volume, then enforce_positivity_highorder:
+ *   SSP-RK3  time_step_ssp_rk_33 (solve_explicit.c:252-301, Gottlieb 2001 eq. 4.2)
+ *   Euler    time_step_euler (solve_explicit.c:226-250)
+ *   LS-RK54  time_step_ls_rk_54 (solve_explicit.c:303-350): sp = a sp + dt r, s += b sp */
 static int rk_elem(const orc_ctx* c, int stage, double dt, int64_t e, double* u, double* u_p, const double* rhs,
                    double* sb)
 {
@@ -881,7 +895,13 @@ static int rk_elem(const orc_ctx* c, int stage, double dt, int64_t e, double* u,
     double* sp = u_p + e * nv * Ns;
     const double* r = rhs + e * nv * Ns;
     for (int i = 0; i < nv * Ns; ++i) {
-        if (stage == 0) {
+        if (c->time_stepper == 1) {
+            s[i] += dt * r[i];
+        } else if (c->time_stepper == 2) {
+            sp[i] *= k_ls54_a[stage];
+            sp[i] += dt * r[i];
+            s[i] += k_ls54_b[stage] * sp[i];
+        } else if (stage == 0) {
             sp[i] = s[i];
             s[i] += dt * r[i];
         } else if (stage == 1) {
@@ -891,6 +911,19 @@ static int rk_elem(const orc_ctx* c, int stage, double dt, int64_t e, double* u,
         }
     }
     return !c->positivity || positivity_elem(c, s, sb);
+}
+
+int orc_set_time_stepper(orc_ctx* c, int time_stepper)
+{
+    if (time_stepper < 0 || time_stepper > 2)
+        return 1;
+    c->time_stepper = time_stepper;
+    return 0;
+}
+
+int orc_n_stages(const orc_ctx* c)
+{
+    return c->time_stepper == 1 ? 1 : (c->time_stepper == 2 ? 5 : 3);
 }
 
 void orc_set_threads(orc_ctx* c, int nthreads)

@@ -88,8 +88,14 @@ void orc_jvp_cs(orc_ctx* c, const double* u, const double* w, double h, double* 
  * first element whose average density/pressure is below EPS_PHYS. */
 int64_t orc_positivity(orc_ctx* c, double* u);
 
-/* One SSP-RK3 stage on (u, u_p) in place; returns max_rhs of the stage's residual.  *bad_elem is set
- * to the first positivity failure (or -1). */
+/* Explicit time stepper of orc_rk_stage (test_case->solver_type_e): 0 SSP-RK3 (default), 1 forward
+ * Euler, 2 low-storage RK 5-stage 4th order (solve_explicit.c:226-350).  Returns 0, or 1 for a bad value. */
+int orc_set_time_stepper(orc_ctx* c, int time_stepper);
+/* Stages per step of the current time stepper (3, 1 or 5). */
+int orc_n_stages(const orc_ctx* c);
+
+/* One stage of the time stepper on (u, u_p) in place (u_p: sol_coef_p, the RK register); returns
+ * max_rhs of the stage's residual.  *bad_elem is set to the first positivity failure (or -1). */
 double orc_rk_stage(orc_ctx* c, int stage, double dt, double* u, double* u_p, int64_t* bad_elem);
 
 /* Physical flux / numerical flux at one point (d = ctx d): used by the flux unit tests.

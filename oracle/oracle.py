@@ -36,6 +36,8 @@ _SIGS = {
     "orc_jvp_cs": (None, [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp, ctypes.c_int]),
     "orc_positivity": (ctypes.c_int64, [ctypes.c_void_p, _dp]),
     "orc_rk_stage": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _i64p]),
+    "orc_set_time_stepper": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "orc_n_stages": (ctypes.c_int, [ctypes.c_void_p]),
     "orc_flux_point": (None, [ctypes.c_int, _dp, _dp]),
     "orc_numflux_point": (None, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
 }
@@ -209,10 +211,20 @@ class Oracle:
         mx = load().orc_rk_stage(self._ptr, stage, dt, _d(u), _d(up), ctypes.byref(bad))
         return mx, bad.value
 
+    def set_time_stepper(self, scheme):
+        """0 SSP-RK3, 1 forward Euler, 2 LS-RK54 (or the names "ssp_rk_33", "euler", "ls_rk_54")."""
+        scheme = {"ssp_rk_33": 0, "euler": 1, "ls_rk_54": 2}.get(scheme, scheme)
+        if load().orc_set_time_stepper(self._ptr, int(scheme)) != 0:
+            raise ValueError(f"bad time stepper {scheme}")
+
+    @property
+    def n_stages(self):
+        return load().orc_n_stages(self._ptr)
+
     def step(self, dt, u, up, nsteps=1):
         mx = 0.0
         for _ in range(nsteps):
-            for s in range(3):
+            for s in range(self.n_stages):
                 mx, bad = self.rk_stage(s, dt, u, up)
                 if bad >= 0:
                     raise RuntimeError(f"positivity failure in element {bad}")

@@ -154,3 +154,28 @@ def test_block_inverse_f32_in_place(n):
         A = shift[b] * np.eye(n) - B[b].astype(np.float64)
         ref = np.linalg.inv(A)
         assert np.linalg.norm(X[b] - ref) / np.linalg.norm(ref) <= 2e-6 * np.linalg.cond(A)
+
+
+def test_launch_split_over_2p20_blocks():
+    """More blocks than one launch takes (2^20 + 5 workgroups): every block of every launch is
+    inverted and applied (1 x 1 blocks keep it cheap), including those past the first launch."""
+    _require_gpu()
+    import torch
+    nb = (1 << 20) + 5
+    print(f"launch split nb={nb}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    B = torch.rand((nb, 1, 1), dtype=torch.float64, device="cuda", generator=g)
+    shift = 2.0 + torch.rand(nb, dtype=torch.float64, device="cuda", generator=g)
+    X = implicit.block_inverse(B, shift)
+    ref = (1.0 / (shift - B[:, 0, 0])).float()
+    assert torch.allclose(X[:, 0, 0], ref, rtol=2e-7, atol=0)
+    x = torch.rand(nb, dtype=torch.float64, device="cuda", generator=g)
+    y = implicit.block_apply(X, x)
+    assert torch.allclose(y, X[:, 0, 0].double() * x.float().double(), rtol=1e-6, atol=0)
+    # the last blocks only (the second launch of a selection)
+    sel = torch.arange(nb - 7, nb, dtype=torch.int32, device="cuda")
+    sel_all = torch.cat([torch.arange(0, 1 << 20, dtype=torch.int32, device="cuda"), sel])
+    y2 = torch.zeros_like(x)
+    implicit.block_apply_sel(X, sel_all, x, y2)
+    torch.cuda.synchronize()
+    assert torch.equal(y2[nb - 7:], y[nb - 7:])

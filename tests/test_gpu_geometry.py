@@ -6,7 +6,11 @@ itself pinned to the oracle's geometry and the reference's metric definitions).
   array's largest magnitude (the two builds sum in different orders: direct sums here, sum
   factorization on the host); host and device output pointers agree bit for bit;
 - the residual of a solver built on the GPU geometry equals the one built on the host geometry;
-- an inverted volume is reported (DGX_ERR_GEOMETRY), p = 5 is refused.
+- an inverted volume is reported (DGX_ERR_GEOMETRY), p = 5 is refused;
+- against the oracle's own geometry (the reference's construction from absolute coordinates) directly,
+  to 1e-12 of each array's largest magnitude (the oracle's absolute-coordinate round-off, see
+  tests/test_host_setup.py);
+- more volumes than one launch takes (2^20 workgroups): every volume is written.
 """
 import math
 import os
@@ -92,3 +96,32 @@ def test_errors():
     with pytest.raises(dg._ffi.DgxError) as ei:
         dg.geometry_gpu(bad, 2)
     assert ei.value.status == 5
+
+
+@pytest.mark.parametrize("d,p,n", [(2, 2, (4, 3)), (2, 3, (3, 3)), (3, 2, (2, 3, 2)), (3, 3, (3, 3, 3)),
+                                   (3, 4, (2, 2, 3))])
+def test_matches_oracle_geometry(d, p, n):
+    _require_gpu()
+    from oracle.oracle import Oracle
+    L = 2 * math.pi
+    mesh = dg.structured_mesh(d, n, (0,) * d, (L,) * d, p_g=p, amp=0.05, seed=3)
+    print(f"geometry vs oracle d={d} p={p} n={n}", flush=True)
+    g = dg.geometry_gpu(mesh, p)
+    m, j, nrm, jf = Oracle(mesh, p).geometry()
+    assert np.abs(g.metrics_vc - m).max() <= 1e-12 * np.abs(m).max()
+    assert np.abs(g.jdet_vc - j).max() <= 1e-12 * np.abs(j).max()
+    assert np.abs(g.normals_fc - nrm).max() <= 1e-12
+    assert np.abs(g.jdet_fc - jf).max() <= 1e-12 * np.abs(jf).max()
+
+
+def test_launch_split_over_2p20_volumes():
+    """1025 x 1025 QUAD p=1 volumes (more than 2^20 workgroups of one launch) against the host build."""
+    _require_gpu()
+    n = 1025
+    mesh = dg.structured_mesh(2, (n, n), (0, 0), (1.0, 1.0), p_g=1, amp=0.0, seed=0)
+    print(f"geometry launch split E={mesh.n_elem}", flush=True)
+    h = dg.geometry(mesh, 1)
+    g = dg.geometry_gpu(mesh, 1)
+    for a, b in ((g.metrics_vc, h.metrics_vc), (g.jdet_vc, h.jdet_vc), (g.normals_fc, h.normals_fc),
+                 (g.jdet_fc, h.jdet_fc)):
+        assert _close(a, b), np.abs(a - b).max()

@@ -149,13 +149,67 @@ def test_create_reports_errors_without_aborting():
     lib = _ffi.load()
     cfg = _ffi.DgxConfig(2, 9, 0, 0, 1, 0)           # p = 9 is not supported
     ctx = ctypes.c_void_p()
-    st = lib.dgx_create(ctypes.byref(cfg), ctypes.byref(mesh.c_struct()), ctypes.byref(geo.c_struct()),
+    st = lib.dgx_create(ctypes.byref(cfg), ctypes.byref(mesh.c_struct()), ctypes.byref(geo.c_struct()), None,
+                        ctypes.byref(ctx))
+    assert st == 1 and not ctx.value
+    cfg = _ffi.DgxConfig(2, 2, 0, 0, 1, 0)
+    cfg.solver_type_e = 3                             # no such time stepper
+    st = lib.dgx_create(ctypes.byref(cfg), ctypes.byref(mesh.c_struct()), ctypes.byref(geo.c_struct()), None,
                         ctypes.byref(ctx))
     assert st == 1 and not ctx.value
     import torch
     if not torch.cuda.is_available():
-        cfg = _ffi.DgxConfig(2, 2, 0, 0, 1, 0)
-        st = lib.dgx_create(ctypes.byref(cfg), ctypes.byref(mesh.c_struct()), ctypes.byref(geo.c_struct()),
-                            ctypes.byref(ctx))
-        assert st == 6 and not ctx.value          # DGX_ERR_NO_DEVICE
+        for ts in (0, 1, 2):
+            cfg = _ffi.DgxConfig(2, 2, 0, 0, 1, 0)
+            cfg.solver_type_e = ts
+            st = lib.dgx_create(ctypes.byref(cfg), ctypes.byref(mesh.c_struct()), ctypes.byref(geo.c_struct()),
+                                None, ctypes.byref(ctx))
+            assert st == 6 and not ctx.value      # DGX_ERR_NO_DEVICE
     assert lib.dgx_status_string(3) == b"DGX_ERR_POSITIVITY"
+
+
+def _ops(p):
+    ops = _ffi.DgxOperators()
+    ops.node_type, ops.kron_order = _ffi.NODES_GL, _ffi.KRON_R_FASTEST
+    assert _ffi.load().dgx_ops1d_build(p, ctypes.byref(ops.line)) == 0
+    return ops
+
+
+def test_create_checks_operator_set():
+    """dgx_create accepts the GL/GL operator set of order p (the reference's LINE factors: reproduced
+    from the oracle's Vandermonde route to round-off) and refuses GLL nodes, another order, another
+    Kronecker tag or a perturbed entry, before any device work."""
+    import torch
+    mesh, geo, u0 = dg.vortex_2d(4, 2)
+    lib = _ffi.load()
+    cfg = _ffi.DgxConfig(2, 2, 0, 0, 1, 0)
+    ctx = ctypes.c_void_p()
+
+    def create(ops):
+        return lib.dgx_create(ctypes.byref(cfg), ctypes.byref(mesh.c_struct()), ctypes.byref(geo.c_struct()),
+                              ctypes.byref(ops), ctypes.byref(ctx))
+
+    # the oracle's operators of order 2 (Vandermonde solves) written into the struct are accepted
+    ops = _ops(2)
+    orc_x, orc_w = O.nodes_1d(0, 2)
+    for i in range(3):
+        ops.line.x[i], ops.line.w[i] = orc_x[i], orc_w[i]
+    ok = 6 if not torch.cuda.is_available() else 0
+    st = create(ops)
+    assert st == ok
+    if ctx.value:
+        lib.dgx_destroy(ctx)
+        ctx = ctypes.c_void_p()
+    bad = []
+    b = _ops(2); b.node_type = _ffi.NODES_GLL; bad.append(b)
+    b = _ops(2); b.kron_order = 1; bad.append(b)
+    b = _ops(3); bad.append(b)
+    b = _ops(2); b.line.D[4] += 1e-9; bad.append(b)
+    b = _ops(2)
+    xg, wg = np.empty(3), np.empty(3)
+    assert lib.dgx_nodes_1d(_ffi.NODES_GLL, 2, _ffi.dptr(xg), _ffi.dptr(wg)) == 0
+    for i in range(3):
+        b.line.x[i], b.line.w[i] = xg[i], wg[i]
+    bad.append(b)
+    for b in bad:
+        assert create(b) == 1 and not ctx.value

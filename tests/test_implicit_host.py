@@ -127,6 +127,9 @@ class _OracleStandIn:
         out.copy_(torch.from_numpy(r))
         return out, mx
 
+    def max_rhs(self, apply_minv=True):
+        return self.o.residual(self.u, apply_minv)[1]
+
     def jvp(self, w, apply_minv=False, out=None):
         out.copy_(torch.from_numpy(self.o.jvp_cs(self.u, w.numpy(), apply_minv=apply_minv)))
         return out
@@ -164,8 +167,31 @@ def test_pseudo_transient_loop_converges_on_standin():
     mx = [h[0] for h in hist]
     assert mx[-1] < 1e-10, mx
     assert hist[0][1] == 1.0 and hist[1][1] > 1.0          # the CFL ramps with max_rhs_0 / max_rhs
-    r, mxo = s.o.residual(uf.numpy())
-    assert mxo < 1e-10
+    r, mxo = s.o.residual(uf.numpy(), apply_minv=False)
+    assert mxo < 1e-10 and mxo == mx[-1]
     # the steady state of a periodic domain near a uniform flow is that uniform flow
     dev = np.abs(uf.numpy() - uf.numpy().mean(axis=(0, 2), keepdims=True)).max()
     assert dev < 1e-6, dev
+
+
+def test_implicit_loop_order_and_max_rhs_as_reference():
+    """The loop of solve_implicit.c:95-108: max_rhs is the un-scaled residual's (before M^-1, the value
+    compute_rlhs_dg returns), the CFL ramps with max_rhs_0 / max_rhs of it, and the step whose max_rhs
+    passes the exit test still solves and applies its update before the loop stops."""
+    mesh, geo, u = _uniform_case(4, 1)
+    s = _OracleStandIn(mesh, geo, 1, 1)
+    s.set_state(torch.from_numpy(u))
+    _, h3 = implicit.solve_implicit(s, mesh, 1, cfl_initial=1.0, exit_tol=0.0, max_steps=3, device="cpu")
+    assert len(h3) == 4 and all(h[2] > 0 for h in h3[:3])
+    assert h3[0][0] == s.o.residual(u, apply_minv=False)[1]          # not the M^-1 residual's
+    assert h3[0][0] != s.o.residual(u, apply_minv=True)[1]
+    for k in range(3):
+        assert h3[k][1] == max(1.0, h3[0][0] / h3[k][0])
+    # an exit tolerance between the first and second step's max_rhs: the second step passes, and its
+    # update is applied (two updates), so the final max_rhs is the third value of the 3-step run
+    tol = 0.5 * (h3[0][0] + h3[1][0])
+    assert h3[1][0] < tol < h3[0][0]
+    s.set_state(torch.from_numpy(u))
+    uf, h2 = implicit.solve_implicit(s, mesh, 1, cfl_initial=1.0, exit_tol=tol, max_steps=10, device="cpu")
+    assert len(h2) == 3 and h2[0][:2] == h3[0][:2] and h2[1][:2] == h3[1][:2]
+    assert abs(h2[-1][0] - h3[2][0]) <= 1e-12 * h3[2][0]
