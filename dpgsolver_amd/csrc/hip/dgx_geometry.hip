@@ -4,8 +4,9 @@
 //
 // The volume's geometry nodes (element-local coordinates, as on the host) sit in LDS; each thread owns
 // one geometry node for the derivative and curl phases and one cubature node for the interpolation
-// phase.  Faces are computed by their left volume from its metric terms (the host's grouping, built
-// here as a CSR list on the host: integer bookkeeping over the face list).
+// phase.  Faces are computed by their left volume from its metric terms (the host's grouping): the table
+// vface[e][lf] holds the face whose left volume is e on its local face lf (-1 otherwise), scattered on
+// the device from the face records.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,14 +53,12 @@ __device__ inline double pass_line(const double* A, const double* in, int i, int
 }
 
 // Wf[(lf * Ng + a) * Nf + q]: interpolation weight of geometry node a at face node q of local face lf
-// (tangential directions in order), products of the 1-D operators built once on the host, a-major so
+// (tangential directions in order), products of the 1-D operators (face_weight_kernel, once per build), a-major so
 // that consecutive threads (face nodes) read consecutive weights.
 template <int D, int VM>
 __global__ __launch_bounds__(BLOCK) void geometry_kernel(GeoOps gk, const double* __restrict__ xyz_g, int64_t E,
                                                          const double* __restrict__ Wf,
-                                                         const int64_t* __restrict__ ffirst,
-                                                         const int64_t* __restrict__ flist,
-                                                         const int8_t* __restrict__ face_lf,
+                                                         const int32_t* __restrict__ vface,
                                                          double* __restrict__ met, double* __restrict__ jdet,
                                                          double* __restrict__ nrm, double* __restrict__ jdf,
                                                          int* __restrict__ bad, int64_t base)
@@ -136,13 +135,13 @@ __global__ __launch_bounds__(BLOCK) void geometry_kernel(GeoOps gk, const double
     }
     __syncthreads();
 
-    // faces whose left volume is e, from the metric terms at the geometry nodes: thread t serves face
-    // ffirst[e] + t / Nf, node t % Nf
-    const int64_t f0 = ffirst[e], nfe = ffirst[e + 1] - f0;
-    for (int64_t u = t; u < nfe * Nf; u += BLOCK) {
-        const int64_t f = flist[f0 + u / Nf];
-        const int q = (int)(u % Nf);
-        const int lf = face_lf[2 * f];
+    // faces whose left volume is e, from the metric terms at the geometry nodes: thread t serves local
+    // face t / Nf, node t % Nf, when e is that face's left volume
+    for (int u = t; u < 2 * D * Nf; u += BLOCK) {
+        const int lf = u / Nf, q = u - lf * Nf;
+        const int64_t f = vface[e * 2 * D + lf];
+        if (f < 0)
+            continue;
         const int dn = lf / 2;
         const double sgn = (lf & 1) ? 1.0 : -1.0;
         double v[D];
@@ -247,28 +246,137 @@ __global__ __launch_bounds__(BLOCK) void geometry_kernel(GeoOps gk, const double
 #undef SW
 }
 
+// Wf[(lf * Ng + a) * Nf + q] (see geometry_kernel), one thread per entry, the factors multiplied in
+// direction order
+__global__ void face_weight_kernel(GeoOps g, int D, double* __restrict__ wf)
+{
+    const int n = 2 * D * g.Ng * g.Nf;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n)
+        return;
+    const int ng = g.ng, nc = g.nc;
+    const int q = idx % g.Nf, a = (idx / g.Nf) % g.Ng, lf = idx / (g.Nf * g.Ng);
+    const int ia[3] = { a % ng, (a / ng) % ng, a / (ng * ng) };
+    int iq[3] = { 0, 0, 0 }, r = q;
+    for (int k = 0; k < D; ++k)
+        if (k != lf / 2) {
+            iq[k] = r % nc;
+            r /= nc;
+        }
+    double v = 1.0;
+    for (int k = 0; k < D; ++k)
+        v *= (k == lf / 2) ? ((lf & 1) ? g.Igp : g.Igm)[ia[k]] : g.Igc[iq[k] * NMAX + ia[k]];
+    wf[idx] = v;
+}
+
+// grid-stride fill (launched with at most 65536 workgroups)
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// vface[e][lf] = f for the left side (face_elem[2f], face_lf[2f]) of every face; a record out of range or
+// a (volume, local face) claimed twice sets bit 2 of *bad.  vface must hold -1 everywhere before.
+__global__ void face_table_kernel(const int32_t* __restrict__ face_elem, const int8_t* __restrict__ face_lf,
+                                  int64_t F, int64_t E, int D, int32_t* __restrict__ vface, int* __restrict__ bad)
+{
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F)
+        return;
+    const int64_t eL = face_elem[2 * f];
+    const int lf = face_lf[2 * f];
+    if (eL < 0 || eL >= E || lf < 0 || lf >= 2 * D) {
+        atomicOr(bad, 2);
+        return;
+    }
+    if (atomicCAS(&vface[eL * 2 * D + lf], -1, (int32_t)f) != -1)
+        atomicOr(bad, 2);
+}
+
+// The whole device pipeline of one geometry build on `st` (no allocation, no synchronisation): the weight
+// table into wf, the face table into vface, *bad cleared and then set (bit 1: det J <= 0, bit 2: bad
+// face record), the metric terms, det J, normals and face det J into om, oj, on, of.
+hipError_t run_geometry(int D, const GeoOps& g, const double* x, int64_t E, const int32_t* face_elem,
+                        const int8_t* face_lf, int64_t F, double* wf, int32_t* vface, int* bad, double* om, double* oj,
+                        double* on, double* of, hipStream_t st);
+
 template <int D>
-void launch_geometry(bool small, const GeoOps& g, const double* x, int64_t E, const double* wf, const int64_t* first,
-                     const int64_t* list, const int8_t* lf, double* om, double* oj, double* on, double* of, int* bad,
-                     hipStream_t st)
+void launch_geometry(bool small, const GeoOps& g, const double* x, int64_t E, const double* wf, const int32_t* vface,
+                     double* om, double* oj, double* on, double* of, int* bad, hipStream_t st)
 {
     // at most 2^20 workgroups per dispatch (a dispatch holds at most 2^32 - 1 work-items)
     const int64_t kMaxGroups = int64_t(1) << 20;
     for (int64_t base = 0; base < E; base += kMaxGroups) {
         const unsigned cnt = (unsigned)((E - base < kMaxGroups) ? E - base : kMaxGroups);
         if (small)
-            hipLaunchKernelGGL((geometry_kernel<D, 64>), dim3(cnt), dim3(BLOCK), 0, st, g, x, E, wf, first, list, lf,
-                               om, oj, on, of, bad, base);
+            hipLaunchKernelGGL((geometry_kernel<D, 64>), dim3(cnt), dim3(BLOCK), 0, st, g, x, E, wf, vface, om, oj,
+                               on, of, bad, base);
         else
-            hipLaunchKernelGGL((geometry_kernel<D, VMAX>), dim3(cnt), dim3(BLOCK), 0, st, g, x, E, wf, first, list,
-                               lf, om, oj, on, of, bad, base);
+            hipLaunchKernelGGL((geometry_kernel<D, VMAX>), dim3(cnt), dim3(BLOCK), 0, st, g, x, E, wf, vface, om, oj,
+                               on, of, bad, base);
     }
+}
+
+hipError_t run_geometry(int D, const GeoOps& g, const double* x, int64_t E, const int32_t* face_elem,
+                        const int8_t* face_lf, int64_t F, double* wf, int32_t* vface, int* bad, double* om, double* oj,
+                        double* on, double* of, hipStream_t st)
+{
+    const int nw = 2 * D * g.Ng * g.Nf;
+    hipLaunchKernelGGL(face_weight_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, g, D, wf);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, st, (int32_t*)bad, (int64_t)1, 0);
+    const int64_t nv = E * 2 * D;
+    const int64_t nb = (nv + 255) / 256;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(256), 0, st, vface, nv, -1);
+    if (F > 0)
+        hipLaunchKernelGGL(face_table_kernel, dim3((unsigned)((F + 255) / 256)), dim3(256), 0, st, face_elem, face_lf,
+                           F, E, D, vface, bad);
+    // grids of at most 64 points (orders up to 3 in 3-D, 7 in 2-D) use the smaller LDS footprint
+    const int vm = std::max(g.Ng, std::max(g.Nc, g.nc * g.ng * (D == 3 ? g.ng : 1)));
+    const bool small = vm <= 64;
+    if (D == 2)
+        launch_geometry<2>(small, g, x, E, wf, vface, om, oj, on, of, bad, st);
+    else
+        launch_geometry<3>(small, g, x, E, wf, vface, om, oj, on, of, bad, st);
+    return hipGetLastError();
 }
 
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() { if (p) (void)hipFree(p); }
 };
+
+// The 1-D operators of a build: GLL geometry nodes of order p_g, GL cubature of order p (host, stack only).
+dgx_status make_geo_ops(int d, int p, int p_g, GeoOps* out)
+{
+    GeoOps& g = *out;
+    std::memset(&g, 0, sizeof(g));
+    g.ng = p_g + 1;
+    g.nc = p + 1;
+    g.Ng = g.ng * g.ng * (d == 3 ? g.ng : 1);
+    g.Nc = g.nc * g.nc * (d == 3 ? g.nc : 1);
+    g.Nf = d == 3 ? g.nc * g.nc : g.nc;
+    double xg[8], wg[8], xc[8], wc[8], tmp[64];
+    const double m1 = -1.0, p1 = 1.0;
+    if (dgx_nodes_1d(DGX_NODES_GLL, p_g, xg, wg) != DGX_OK || dgx_nodes_1d(DGX_NODES_GL, p, xc, wc) != DGX_OK)
+        return DGX_ERR_INVALID_ARG;
+    const int ng = g.ng, nc = g.nc;
+    dgx_lagrange_1d(ng, xg, ng, xg, 1, tmp);
+    for (int a = 0; a < ng; ++a)
+        for (int m = 0; m < ng; ++m)
+            g.Dg[a * NMAX + m] = tmp[a * ng + m];
+    dgx_lagrange_1d(ng, xg, nc, xc, 0, tmp);
+    for (int n = 0; n < nc; ++n)
+        for (int m = 0; m < ng; ++m)
+            g.Igc[n * NMAX + m] = tmp[n * ng + m];
+    dgx_lagrange_1d(ng, xg, nc, xc, 1, tmp);
+    for (int n = 0; n < nc; ++n)
+        for (int m = 0; m < ng; ++m)
+            g.Dgc[n * NMAX + m] = tmp[n * ng + m];
+    dgx_lagrange_1d(ng, xg, 1, &m1, 0, g.Igm);
+    dgx_lagrange_1d(ng, xg, 1, &p1, 0, g.Igp);
+    return DGX_OK;
+}
 
 }  // namespace
 }  // namespace dgx
@@ -280,7 +388,7 @@ extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int pt
     if (!mesh || !mesh->xyz_g || !mesh->face_elem || !mesh->face_lf || !metrics_vc || !jdet_vc ||
         !normals_fc || !jdet_fc || (mesh->d != 2 && mesh->d != 3) || p < 1 || p > NMAX - 1 ||
         mesh->p_g < 1 || mesh->p_g > NMAX - 1 || mesh->n_elem < 1 || mesh->n_elem > 0x7fffffff ||
-        mesh->n_face < 0 || (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
+        mesh->n_face < 0 || mesh->n_face > 0x7fffffff || (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE))
         return DGX_ERR_INVALID_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -289,79 +397,21 @@ extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int pt
     const int64_t E = mesh->n_elem, F = mesh->n_face;
 
     GeoOps g;
-    std::memset(&g, 0, sizeof(g));
-    g.ng = mesh->p_g + 1;
-    g.nc = p + 1;
-    g.Ng = g.ng * g.ng * (d == 3 ? g.ng : 1);
-    g.Nc = g.nc * g.nc * (d == 3 ? g.nc : 1);
-    g.Nf = d == 3 ? g.nc * g.nc : g.nc;
-    {
-        double xg[8], wg[8], xc[8], wc[8], tmp[64];
-        const double m1 = -1.0, p1 = 1.0;
-        if (dgx_nodes_1d(DGX_NODES_GLL, mesh->p_g, xg, wg) != DGX_OK ||
-            dgx_nodes_1d(DGX_NODES_GL, p, xc, wc) != DGX_OK)
-            return DGX_ERR_INVALID_ARG;
-        const int ng = g.ng, nc = g.nc;
-        dgx_lagrange_1d(ng, xg, ng, xg, 1, tmp);
-        for (int a = 0; a < ng; ++a)
-            for (int m = 0; m < ng; ++m)
-                g.Dg[a * NMAX + m] = tmp[a * ng + m];
-        dgx_lagrange_1d(ng, xg, nc, xc, 0, tmp);
-        for (int n = 0; n < nc; ++n)
-            for (int m = 0; m < ng; ++m)
-                g.Igc[n * NMAX + m] = tmp[n * ng + m];
-        dgx_lagrange_1d(ng, xg, nc, xc, 1, tmp);
-        for (int n = 0; n < nc; ++n)
-            for (int m = 0; m < ng; ++m)
-                g.Dgc[n * NMAX + m] = tmp[n * ng + m];
-        dgx_lagrange_1d(ng, xg, 1, &m1, 0, g.Igm);
-        dgx_lagrange_1d(ng, xg, 1, &p1, 0, g.Igp);
-    }
-
-    // face weight table (see geometry_kernel)
-    const int Ng = g.Ng, Nf = g.Nf, ng = g.ng, nc = g.nc;
-    std::vector<double> wf((size_t)2 * d * Ng * Nf);
-    for (int lf = 0; lf < 2 * d; ++lf)
-        for (int a = 0; a < Ng; ++a)
-            for (int q = 0; q < Nf; ++q) {
-                const int ia[3] = { a % ng, (a / ng) % ng, a / (ng * ng) };
-                int iq[3] = { 0, 0, 0 }, r = q;
-                for (int k = 0; k < d; ++k)
-                    if (k != lf / 2) {
-                        iq[k] = r % nc;
-                        r /= nc;
-                    }
-                double v = 1.0;
-                for (int k = 0; k < d; ++k)
-                    v *= (k == lf / 2) ? ((lf & 1) ? g.Igp : g.Igm)[ia[k]] : g.Igc[iq[k] * NMAX + ia[k]];
-                wf[((size_t)lf * Ng + a) * Nf + q] = v;
-            }
-
-    // faces grouped by their left volume (CSR)
-    std::vector<int64_t> first((size_t)E + 1, 0), list((size_t)(F > 0 ? F : 1));
+    if (make_geo_ops(d, p, mesh->p_g, &g) != DGX_OK)
+        return DGX_ERR_INVALID_ARG;
+    // face records: the left side of every face in range (the device scatter checks the rest)
     for (int64_t f = 0; f < F; ++f) {
         const int64_t eL = mesh->face_elem[2 * f];
         const int lf = mesh->face_lf[2 * f];
         if (eL < 0 || eL >= E || lf < 0 || lf >= 2 * d)
             return DGX_ERR_INVALID_ARG;
-        first[(size_t)eL + 1]++;
     }
-    for (int64_t e = 0; e < E; ++e)
-        first[(size_t)e + 1] += first[(size_t)e];
-    {
-        std::vector<int64_t> fill(first.begin(), first.end() - 1);
-        for (int64_t f = 0; f < F; ++f)
-            list[(size_t)fill[(size_t)mesh->face_elem[2 * f]]++] = f;
-    }
-    std::vector<int8_t> lfs((size_t)(2 * (F > 0 ? F : 1)), 0);
-    for (int64_t f = 0; f < 2 * F; ++f)
-        lfs[(size_t)f] = mesh->face_lf[f];
-
     const size_t b_xyz = sizeof(double) * (size_t)E * d * g.Ng;
     const size_t b_met = sizeof(double) * (size_t)E * d * d * g.Nc, b_jd = sizeof(double) * (size_t)E * g.Nc;
     const size_t b_nrm = sizeof(double) * (size_t)(F > 0 ? F : 1) * g.Nf * d;
     const size_t b_jdf = sizeof(double) * (size_t)(F > 0 ? F : 1) * g.Nf;
-    DevBuf dx, dwf, dfirst, dlist, dlf, dbad, dmet, djd, dnrm, djdf;
+    const size_t b_wf = sizeof(double) * (size_t)2 * d * g.Ng * g.Nf;
+    DevBuf dx, dwf, dvface, dfe, dlf, dbad, dmet, djd, dnrm, djdf;
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     dgx_status rc = DGX_OK;
@@ -377,10 +427,10 @@ extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int pt
     GEO_TRY(hipEventCreate(&ev0));
     GEO_TRY(hipEventCreate(&ev1));
     GEO_TRY(hipMalloc(&dx.p, b_xyz));
-    GEO_TRY(hipMalloc(&dwf.p, sizeof(double) * wf.size()));
-    GEO_TRY(hipMalloc(&dfirst.p, sizeof(int64_t) * first.size()));
-    GEO_TRY(hipMalloc(&dlist.p, sizeof(int64_t) * list.size()));
-    GEO_TRY(hipMalloc(&dlf.p, lfs.size()));
+    GEO_TRY(hipMalloc(&dwf.p, b_wf));
+    GEO_TRY(hipMalloc(&dvface.p, sizeof(int32_t) * (size_t)E * 2 * d));
+    GEO_TRY(hipMalloc(&dfe.p, sizeof(int32_t) * 2 * (size_t)(F > 0 ? F : 1)));
+    GEO_TRY(hipMalloc(&dlf.p, 2 * (size_t)(F > 0 ? F : 1)));
     GEO_TRY(hipMalloc(&dbad.p, sizeof(int)));
     if (ptr_kind == DGX_PTR_HOST) {
         GEO_TRY(hipMalloc(&dmet.p, b_met));
@@ -394,22 +444,13 @@ extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int pt
         double* on = ptr_kind == DGX_PTR_HOST ? (double*)dnrm.p : normals_fc;
         double* of = ptr_kind == DGX_PTR_HOST ? (double*)djdf.p : jdet_fc;
         GEO_TRY(hipMemcpyAsync(dx.p, mesh->xyz_g, b_xyz, hipMemcpyHostToDevice, st));
-        GEO_TRY(hipMemcpyAsync(dwf.p, wf.data(), sizeof(double) * wf.size(), hipMemcpyHostToDevice, st));
-        GEO_TRY(hipMemcpyAsync(dfirst.p, first.data(), sizeof(int64_t) * first.size(), hipMemcpyHostToDevice, st));
-        GEO_TRY(hipMemcpyAsync(dlist.p, list.data(), sizeof(int64_t) * list.size(), hipMemcpyHostToDevice, st));
-        GEO_TRY(hipMemcpyAsync(dlf.p, lfs.data(), lfs.size(), hipMemcpyHostToDevice, st));
-        GEO_TRY(hipMemcpyAsync(dbad.p, &bad, sizeof(int), hipMemcpyHostToDevice, st));
+        if (F > 0) {
+            GEO_TRY(hipMemcpyAsync(dfe.p, mesh->face_elem, sizeof(int32_t) * 2 * F, hipMemcpyHostToDevice, st));
+            GEO_TRY(hipMemcpyAsync(dlf.p, mesh->face_lf, 2 * F, hipMemcpyHostToDevice, st));
+        }
         GEO_TRY(hipEventRecord(ev0, st));
-        // grids of at most 64 points (orders up to 3 in 3-D, 7 in 2-D) use the smaller LDS footprint
-        const int vm = std::max(g.Ng, std::max(g.Nc, g.nc * g.ng * (d == 3 ? g.ng : 1)));
-        const bool small = vm <= 64;
-        if (d == 2)
-            launch_geometry<2>(small, g, (const double*)dx.p, E, (const double*)dwf.p, (const int64_t*)dfirst.p,
-                                (const int64_t*)dlist.p, (const int8_t*)dlf.p, om, oj, on, of, (int*)dbad.p, st);
-        else
-            launch_geometry<3>(small, g, (const double*)dx.p, E, (const double*)dwf.p, (const int64_t*)dfirst.p,
-                                (const int64_t*)dlist.p, (const int8_t*)dlf.p, om, oj, on, of, (int*)dbad.p, st);
-        GEO_TRY(hipGetLastError());
+        GEO_TRY(run_geometry(d, g, (const double*)dx.p, E, (const int32_t*)dfe.p, (const int8_t*)dlf.p, F,
+                             (double*)dwf.p, (int32_t*)dvface.p, (int*)dbad.p, om, oj, on, of, st));
         GEO_TRY(hipEventRecord(ev1, st));
         if (ptr_kind == DGX_PTR_HOST) {
             GEO_TRY(hipMemcpyAsync(metrics_vc, om, b_met, hipMemcpyDeviceToHost, st));
@@ -427,7 +468,9 @@ extern "C" dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int pt
             *kernel_ms = ms;
         }
     }
-    if (bad)
+    if (bad & 2)
+        rc = DGX_ERR_INVALID_ARG;
+    else if (bad)
         rc = DGX_ERR_GEOMETRY;
 done:
 #undef GEO_TRY
