@@ -121,6 +121,9 @@ SIGNATURES = {
                                           c_double_p, c_double_p]),
     "dgx_geometry_build_gpu": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_double_p]),
+    "dgx_geometry_build_gpu_noalloc": (ctypes.c_int, [ctypes.POINTER(DgxMesh), ctypes.c_int, ctypes.c_void_p,
+                                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                      ctypes.c_void_p, ctypes.c_void_p]),
     "dgx_restart_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(DgxMesh), ctypes.c_int, c_double_p]),
     "dgx_restart_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, c_double_p]),
     "dgx_vtu_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(DgxMesh), ctypes.c_int, c_double_p]),

@@ -10,7 +10,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -481,4 +483,71 @@ done:
     if (ev1)
         (void)hipEventDestroy(ev1);
     return rc;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Allocation-free variant.  Its scratch memory (the face weight table and the per-volume face table) is
+// this file-scope array: the loader gives every GPU its own copy, held until the process ends.  In host
+// code `scratch` names a host-side stand-in, so its device address is looked up once per device
+// (hipGetSymbolAddress) and only that address is passed to kernels.  Calls on one GPU share the array:
+// calls that overlap in time on one GPU race.
+// ---------------------------------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchBytes = size_t(32) << 20;   // 32 MiB per GPU
+constexpr size_t kWfBytes = 153600;                   // >= 2 * 3 * 125 * 25 doubles (p, p_g <= 4), 256-aligned
+constexpr int kMaxDevices = 64;
+std::atomic<unsigned char*> g_scratch_dev[kMaxDevices];
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of this device's scratch array (looked up on the first call on each device).
+static unsigned char* scratch_address(int dev)
+{
+    unsigned char* ptr = g_scratch_dev[dev].load(std::memory_order_acquire);
+    if (ptr)
+        return ptr;
+    void* addr = nullptr;
+    if (hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)) != hipSuccess || !addr)
+        return nullptr;
+    g_scratch_dev[dev].store((unsigned char*)addr, std::memory_order_release);
+    return (unsigned char*)addr;
+}
+
+extern "C" dgx_status dgx_geometry_build_gpu_noalloc(const dgx_mesh* mesh, int p, double* metrics_vc,
+                                                     double* jdet_vc, double* normals_fc, double* jdet_fc,
+                                                     int32_t* flags, void* stream)
+{
+    using namespace dgx;
+    if (!mesh || !mesh->xyz_g || !mesh->face_elem || !mesh->face_lf || !metrics_vc || !jdet_vc || !normals_fc ||
+        !jdet_fc || !flags || (mesh->d != 2 && mesh->d != 3) || p < 1 || p > NMAX - 1 || mesh->p_g < 1 ||
+        mesh->p_g > NMAX - 1 || mesh->n_elem < 1 || mesh->n_elem > 0x7fffffff || mesh->n_face < 0 ||
+        mesh->n_face > 0x7fffffff)
+        return DGX_ERR_INVALID_ARG;
+    const int d = mesh->d;
+    const int64_t E = mesh->n_elem, F = mesh->n_face;
+    const size_t need = sizeof(int32_t) * (size_t)E * 2 * d;
+    if (need > kScratchBytes - kWfBytes) {
+        fprintf(stderr,
+                "dgx_geometry_build_gpu_noalloc: %lld volumes need %zu bytes of face table, more than the %zu "
+                "bytes of scratch (at most %zu volumes in %d-D)\n",
+                (long long)E, need, kScratchBytes - kWfBytes, (kScratchBytes - kWfBytes) / (sizeof(int32_t) * 2 * d),
+                d);
+        return DGX_ERR_INVALID_ARG;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0)
+        return DGX_ERR_NO_DEVICE;
+    if (dev >= kMaxDevices)
+        return DGX_ERR_INVALID_ARG;
+    unsigned char* base = scratch_address(dev);
+    if (!base)
+        return DGX_ERR_HIP;
+    GeoOps g;
+    if (make_geo_ops(d, p, mesh->p_g, &g) != DGX_OK)
+        return DGX_ERR_INVALID_ARG;
+    const hipError_t e = run_geometry(d, g, mesh->xyz_g, E, mesh->face_elem, mesh->face_lf, F, (double*)base,
+                                      (int32_t*)(base + kWfBytes), flags, metrics_vc, jdet_vc, normals_fc, jdet_fc,
+                                      (hipStream_t)stream);
+    return e == hipSuccess ? DGX_OK : DGX_ERR_HIP;
 }

@@ -92,6 +92,20 @@ dgx_status dgx_geometry_build(const dgx_mesh* mesh, int p,
 dgx_status dgx_geometry_build_gpu(const dgx_mesh* mesh, int p, int ptr_kind, double* metrics_vc,
                                   double* jdet_vc, double* normals_fc, double* jdet_fc, double* kernel_ms);
 
+/* dgx_geometry_build_gpu that allocates nothing: no reference counterpart.  For callers that cannot
+ * allocate device memory or create streams (HIP graph capture, e.g. a moving-mesh time loop that
+ * rebuilds the geometry every step, or a loop that forbids allocation).  Same kernels and results (bit
+ * for bit) as dgx_geometry_build_gpu.  Every pointer in mesh (xyz_g, face_elem, face_lf) and every
+ * output is a DEVICE pointer; the work is enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * and the call returns without synchronising.  *flags (device int32) is written with 0, or with bit 1
+ * when det J <= 0 at some cubature node and bit 2 when a face record is out of range or two faces claim
+ * one (volume, local face).  Scratch: a 32 MiB device array per GPU held by the library until the
+ * process ends (the face weight table and a face table of 2 d int32 per volume): at most 1,391,701
+ * volumes in 3-D (2,087,552 in 2-D); larger meshes get DGX_ERR_INVALID_ARG, with the limit printed to
+ * stderr.  Calls on one GPU share that scratch: calls that overlap in time on one GPU race. */
+dgx_status dgx_geometry_build_gpu_noalloc(const dgx_mesh* mesh, int p, double* metrics_vc, double* jdet_vc,
+                                          double* normals_fc, double* jdet_fc, int32_t* flags, void* stream);
+
 /* Restart file of the state sol[n_elem][d+2][Ns] (reference layout, host) in the reference's format
  * (output_restart_gmsh, test_case/solution/restart/restart_writers.c:149-420): a Gmsh 2.2 mesh whose
  * nodes are the volumes' vertices sorted (last coordinate first) and made unique within a relative
