@@ -112,10 +112,14 @@ def cpu_baseline(args, d, p, num_flux):
             "single_thread": {"value": r1, "cores": 1, "sample": f"{s1} stages in {e1:.1f} s"}}
 
 
+PROTOCOL_STEPS = 50   # SSP-RK3 steps of the state comparisons of the 64^3 workload
+
+
 def precision_form(args, dg, torch, mesh, geo, u0, dt, fp64_solver, fp64_stage_ms):
-    """The precision form (FP32 hi/lo-split contractions, FP32 accumulation) against the FP64 run just
-    timed: stage time ratio, residual error at the initial state, state error after the same steps,
-    and FP64's own round-off floor (residual change under a 1e-16 relative input perturbation)."""
+    """The precision form (line-differenced FP32 hi/lo contractions, FP32 accumulation) against the FP64
+    run just timed: stage time ratio over the same timed steps, residual error at the initial state,
+    state error after PROTOCOL_STEPS steps from u0 (both forms), and FP64's own round-off floor (residual
+    change under a 1e-16 relative input perturbation)."""
     dev = torch.device("cuda:0")
     ut = torch.from_numpy(u0).to(dev)
     mixed = dg.Solver(mesh, geo, 3, num_flux="roe", precision="mixed")
@@ -124,12 +128,16 @@ def precision_form(args, dg, torch, mesh, geo, u0, dt, fp64_solver, fp64_stage_m
         mixed.step(dt, args.warmup)
     mixed.step(dt, args.steps)
     mixed_stage_ms = mixed.last_elapsed_ms / (3 * args.steps)
-    u_fp64 = fp64_solver.get_state(torch.empty_like(ut))
+    mixed.set_state(ut)
+    mixed.step(dt, PROTOCOL_STEPS)
     u_mixed = mixed.get_state(torch.empty_like(ut))
     mixed.set_state(ut)
     r_mixed = torch.empty_like(ut)
     mixed.compute_rhs(precision="mixed", out=r_mixed)
     mixed.close()
+    fp64_solver.set_state(ut)
+    fp64_solver.step(dt, PROTOCOL_STEPS)
+    u_fp64 = fp64_solver.get_state(torch.empty_like(ut))
     fp64_solver.set_state(ut)
     r64 = torch.empty_like(ut)
     fp64_solver.compute_rhs(out=r64)
@@ -143,13 +151,105 @@ def precision_form(args, dg, torch, mesh, geo, u0, dt, fp64_solver, fp64_stage_m
 
     return {"kernel": "dgx::hex3_stage_kernel<1,true>", "speedup_vs_fp64": fp64_stage_ms / mixed_stage_ms,
             "stage_us": mixed_stage_ms * 1e3, "residual_rel_l2": rel(r_mixed, r64),
-            "state_rel_l2": rel(u_mixed, u_fp64), "state_steps": args.warmup + args.steps,
+            "state_rel_l2": rel(u_mixed, u_fp64), "state_steps": PROTOCOL_STEPS,
             "fp64_roundoff_floor_rel_l2": rel(r_eps, r64), "target_rel_err": 1e-9,
             "meets_target": rel(r_mixed, r64) <= 1e-9}
 
 
+def fp64_vs_oracle(args, dg, torch, mesh, geo, u0, dt, p, num_flux, solver, viscous=None, steps=0, minv=False,
+                   pre_minv=True):
+    """GPU FP64 against the oracle at the benchmark's full size: the pre-M^-1 residual (pre_minv) and the
+    M^-1 residual (minv: the oracle first builds its dense inverse mass matrices) at the initial state, and
+    (steps > 0) the state after that many SSP-RK3 steps, as relative L2 errors, next to the GPU's own
+    round-off floor (the residual change under a 1e-16 relative input perturbation).  The oracle runs on
+    all host threads."""
+    import numpy as np
+    from oracle.oracle import Oracle
+    t0 = time.perf_counter()
+    orc = Oracle(mesh, p, num_flux=num_flux, geometry=geo, viscous=viscous)
+    orc.set_threads(host_threads())
+
+    def rel(a, b):
+        return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+    out = {}
+    solver.set_state(u0)
+    kinds = ([False] if pre_minv else []) + ([True] if minv else [])
+    for mv in kinds:
+        r_gpu, _ = solver.compute_rhs(apply_minv=mv)
+        r_orc, _ = orc.residual(u0, apply_minv=mv)
+        out["residual_rel_l2" if mv else "residual_pre_minv_rel_l2"] = rel(r_gpu, r_orc)
+    rng = np.random.default_rng(0)
+    solver.set_state(u0 * (1 + 1.1e-16 * rng.standard_normal(u0.shape)))
+    r_eps, _ = solver.compute_rhs(apply_minv=kinds[-1])
+    out["gpu_roundoff_floor_rel_l2"] = rel(r_eps, r_gpu)
+    if steps > 0:
+        u, up = u0.copy(), np.zeros_like(u0)
+        orc.step(dt, u, up, steps)
+        solver.set_state(u0)
+        solver.step(dt, steps)
+        out["state_rel_l2"] = rel(solver.get_state(), u)
+        out["state_steps"] = steps
+    out["oracle_threads"] = host_threads()
+    out["seconds"] = time.perf_counter() - t0
+    return out
+
+
 def kinetic_energy(u):
     return float((0.5 * (u[:, 1] ** 2 + u[:, 2] ** 2 + u[:, 3] ** 2) / u[:, 0]).sum())
+
+
+KE_STEPS = 200   # SSP-RK3 steps of the Navier-Stokes kinetic-energy history
+
+
+def ke_history(dg, torch, solver, u0, dt, steps, every):
+    """Kinetic energy sum_nodes rho |u|^2 / 2 (nodal sum over the whole mesh) every `every` SSP-RK3 steps
+    over `steps` steps from u0, computed from the device state."""
+    ut = torch.from_numpy(u0).cuda()
+    solver.set_state(ut)
+    buf = torch.empty_like(ut)
+
+    def ke():
+        solver.get_state(buf)
+        return float((0.5 * (buf[:, 1] ** 2 + buf[:, 2] ** 2 + buf[:, 3] ** 2) / buf[:, 0]).sum())
+
+    hist = [ke()]
+    for _ in range(steps // every):
+        solver.step(dt, every)
+        hist.append(ke())
+    return {"every_steps": every, "steps": steps, "dt": dt, "history": hist}
+
+
+def ns_small_vs_oracle(dg, steps):
+    """The Navier-Stokes protocol (TGV Re=1600, M=0.1, HEX p=4, Roe + BR2, SSP-RK3) at the size the host
+    oracle finishes in about a minute (6^3): state and kinetic-energy error after `steps` steps, next to the
+    GPU round-off floor of the state (the same steps from a 1e-16 relative perturbation of the start)."""
+    import numpy as np
+    from oracle.oracle import Oracle
+    n, p, visc = 6, 4, (1.0 / 1600.0, 0.71)
+    mesh = dg.structured_mesh(3, (n, n, n), (0, 0, 0), (2 * math.pi,) * 3, p_g=1)
+    geo = dg.geometry(mesh, p)
+    u0 = dg.ic_taylor_green(mesh, p, 0.1)
+    dt = dg.dt_cfl(mesh, p, u0, 0.5)
+    orc = Oracle(mesh, p, num_flux=1, geometry=geo, viscous=visc)
+    orc.set_threads(host_threads())
+    u, up = u0.copy(), np.zeros_like(u0)
+    orc.step(dt, u, up, steps)
+    s = dg.Solver(mesh, geo, p, num_flux="roe", viscous=visc)
+    s.set_state(u0)
+    s.step(dt, steps)
+    ug = s.get_state()
+    rng = np.random.default_rng(1)
+    s.set_state(u0 * (1 + 1.1e-16 * rng.standard_normal(u0.shape)))
+    s.step(dt, steps)
+    ue = s.get_state()
+    s.close()
+
+    def rel(a, b):
+        return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+    return {"mesh": f"{n}^3", "steps": steps, "state_rel_l2": rel(ug, u), "gpu_roundoff_floor_rel_l2": rel(ue, ug),
+            "kinetic_energy_rel": abs(kinetic_energy(ug) - kinetic_energy(u)) / kinetic_energy(u)}
 
 
 def bench_vjp(args, dg, torch):
@@ -375,6 +475,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision-form", action="store_true")
+    ap.add_argument("--no-oracle-check", action="store_true",
+                    help="skip the full-size FP64 residual comparison with the oracle")
+    ap.add_argument("--oracle-check", action="store_true",
+                    help="also compare the state after the protocol's steps with the oracle (minutes of host time)")
     ap.add_argument("--cpu-sample-n", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--traffic-file", default=os.path.join(HERE, "profiles", "stage_hex_p3_64_traffic.md"))
@@ -478,11 +582,20 @@ def main():
     }
     if args.config in ("hex64", "hex32") and args.precision == "fp64" and not args.no_precision_form:
         out["precision_form"] = precision_form(args, dg, torch, mesh, geo, u0, dt, solver, ms / stages)
+    if args.config in ("hex64", "hex32") and args.precision == "fp64" and not args.no_oracle_check:
+        # full-size FP64 parity of the pre-M^-1 residual (seconds of host time); the M^-1 residual and the
+        # state after PROTOCOL_STEPS steps (the oracle's stages at full size: minutes) with --oracle-check
+        out["fp64_vs_oracle"] = fp64_vs_oracle(args, dg, torch, mesh, geo, u0, dt, p, num_flux, solver,
+                                               steps=PROTOCOL_STEPS if args.oracle_check else 0,
+                                               minv=args.oracle_check)
+    if args.config == "ns48" and args.oracle_check:
+        # one full-size Navier-Stokes residual (M^-1 applied) through the oracle: minutes of host time
+        out["fp64_vs_oracle"] = fp64_vs_oracle(args, dg, torch, mesh, geo, u0, dt, p, num_flux, solver,
+                                               viscous=viscous, minv=True, pre_minv=False)
     if viscous:
-        # kinetic-energy history of the run (nodal sum of rho |u|^2 / 2 over the whole mesh)
-        uu = solver.get_state()
-        out["kinetic_energy"] = {"initial": kinetic_energy(u0), "final": kinetic_energy(uu),
-                                 "steps": args.warmup + args.steps, "dt": dt}
+        out["kinetic_energy"] = ke_history(dg, torch, solver, u0, dt, KE_STEPS, 10)
+        if args.oracle_check:
+            out["small_mesh_vs_oracle"] = ns_small_vs_oracle(dg, KE_STEPS)
     if not args.no_cpu_baseline and not viscous:
         out["cpu_baseline"] = cpu_baseline(args, d, p, num_flux)
     print(json.dumps(out), flush=True)

@@ -40,6 +40,7 @@ struct dgx_ctx {
     double* imass = nullptr;     // [Ep][ns]
     double* tr[2] = { nullptr, nullptr };   // [Ep][nface][nv][nf]
     uint32_t* nbr = nullptr;     // [Ep][nface]
+    uint8_t* nord = nullptr;     // [Ep][nface] face node correspondence case (null when every face is aligned)
     int32_t* slot_elem = nullptr;   // [Ep]
     int32_t* elem_slot = nullptr;   // [E]
     uint8_t* face_list = nullptr;   // [G][48]
@@ -87,7 +88,7 @@ void free_all(dgx_ctx* c)
     void* ptrs[] = { c->u, c->up, c->rhs, c->stage_buf, c->met, c->imass, c->tr[0], c->tr[1], c->nbr,
                      c->slot_elem, c->elem_slot, c->face_list, c->face_count, c->blockmax, c->dmax,
                      c->bad_elem, c->nonfinite, c->gate, c->lin_x, c->lin_tr, c->gtr[0], c->gtr[1], c->lam,
-                     c->fnv };
+                     c->fnv, c->nord };
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -197,6 +198,7 @@ hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin,
     a.tr_in = c->tr[tin];
     a.tr_out = c->tr[tin ^ 1];
     a.nbr = c->nbr;
+    a.nord = c->nord;
     a.rhs_out = c->rhs;
     a.blockmax = c->blockmax;
     a.bad_elem = c->bad_elem;
@@ -387,14 +389,17 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
 
     // neighbour table in element order: every (element, local face) slot must be filled exactly once
     std::vector<int64_t> nbe((size_t)E * nface, -1);
-    std::vector<int8_t> nbl((size_t)E * nface, -1), nbr_right((size_t)E * nface, 0);
+    std::vector<int8_t> nbl((size_t)E * nface, -1), nbr_right((size_t)E * nface, 0), nbo((size_t)E * nface, 0);
+    bool aligned = true;
     for (int64_t f = 0; f < F; ++f) {
         const int64_t eL = mesh->face_elem[2 * f], eR = mesh->face_elem[2 * f + 1];
         const int lL = mesh->face_lf[2 * f], lR = mesh->face_lf[2 * f + 1];
         if (eL < 0 || eL >= E || eR < 0 || eR >= E || lL < 0 || lL >= nface || lR < 0 || lR >= nface)
             return DGX_ERR_INVALID_ARG;
+        if (mesh->face_ord[f] < 0 || mesh->face_ord[f] > (d == 2 ? 1 : 7))
+            return DGX_ERR_INVALID_ARG;   // LINE faces have 2 correspondence cases, QUAD faces 8
         if (mesh->face_ord[f] != 0)
-            return DGX_ERR_INVALID_ARG;   // only aligned node correspondence is supported
+            aligned = false;
         const size_t sl = eL * nface + lL, sr = eR * nface + lR;
         if (nbe[sl] >= 0 || nbe[sr] >= 0)
             return DGX_ERR_INVALID_ARG;
@@ -404,10 +409,13 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         nbe[sr] = eL;
         nbl[sr] = (int8_t)lL;
         nbr_right[sr] = 1;
+        nbo[sl] = nbo[sr] = mesh->face_ord[f];   // one gather for both sides, as the reference applies nc_fc
     }
     for (int64_t v : nbe)
         if (v < 0)
             return DGX_ERR_INVALID_ARG;   // boundary faces are not part of the periodic explicit path
+    if (!aligned && visc)
+        return DGX_ERR_INVALID_ARG;       // the Navier-Stokes kernels gather aligned faces only
 
     dgx_ctx* c = new dgx_ctx();
     c->d = d;
@@ -429,7 +437,8 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         c->Pr = cfg->Pr;
         c->eta = 1.01 * nface;   // PENALTY_SCALING * NFMAX (compute_face_rlhs_dg_T.c:276-296)
     }
-    c->hex3 = (d == 3 && p == 3 && !visc);
+    // the HEX p=3 group kernel assumes aligned faces; other meshes run the generic kernel of that order
+    c->hex3 = (d == 3 && p == 3 && !visc && aligned);
     if (cfg->precision == DGX_PREC_MIXED && !c->hex3) {
         delete c;
         return DGX_ERR_INVALID_ARG;   // the precision form is built for Euler HEX p=3
@@ -575,6 +584,8 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     step(hipMalloc(&c->nbr, sizeof(uint32_t) * nbr.size()), "hipMalloc(nbr)");
     step(hipMalloc(&c->slot_elem, sizeof(int32_t) * slot_elem.size()), "hipMalloc(slot_elem)");
     step(hipMalloc(&c->elem_slot, sizeof(int32_t) * elem_slot.size()), "hipMalloc(elem_slot)");
+    if (!aligned)
+        step(hipMalloc(&c->nord, (size_t)Ep * nface), "hipMalloc(nord)");
     if (c->hex3) {
         step(hipMalloc(&c->face_list, flist.size()), "hipMalloc(face_list)");
         step(hipMalloc(&c->face_count, fcount.size()), "hipMalloc(face_count)");
@@ -600,6 +611,15 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
              "copy slot_elem");
         step(hipMemcpy(c->elem_slot, elem_slot.data(), sizeof(int32_t) * elem_slot.size(), hipMemcpyHostToDevice),
              "copy elem_slot");
+        if (c->nord) {
+            // slot order (the identity for the generic kernels)
+            std::vector<uint8_t> nord((size_t)Ep * nface, 0);
+            for (int64_t sl = 0; sl < Ep; ++sl)
+                if (slot_elem[sl] >= 0)
+                    for (int f = 0; f < nface; ++f)
+                        nord[sl * nface + f] = (uint8_t)nbo[(size_t)slot_elem[sl] * nface + f];
+            step(hipMemcpy(c->nord, nord.data(), nord.size(), hipMemcpyHostToDevice), "copy nord");
+        }
         if (c->hex3) {
             step(hipMemcpy(c->face_list, flist.data(), flist.size(), hipMemcpyHostToDevice), "copy face_list");
             step(hipMemcpy(c->face_count, fcount.data(), fcount.size(), hipMemcpyHostToDevice), "copy face_count");
@@ -855,6 +875,7 @@ static dgx_status lin_apply(dgx_ctx* c, bool adjoint, int apply_minv, int layout
     a.tr_u = c->tr[c->cur];
     a.tr_x = c->lin_tr;
     a.nbr = c->nbr;
+    a.nord = c->nord;
     a.slot_elem = c->hex3 ? c->slot_elem : nullptr;
     a.out = c->rhs;
     a.n_elem = c->Ep;
@@ -899,6 +920,7 @@ static dgx_status jacobian_blocks(dgx_ctx* c, int apply_minv, double* bt64, floa
     a.imass = c->imass;
     a.tr_u = c->tr[c->cur];
     a.nbr = c->nbr;
+    a.nord = c->nord;
     a.slot_elem = c->hex3 ? c->slot_elem : nullptr;
     a.n_elem = c->Ep;
     a.apply_minv = apply_minv ? 1 : 0;

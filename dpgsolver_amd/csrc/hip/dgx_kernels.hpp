@@ -73,6 +73,32 @@ __device__ inline int face_node(int dn, int a, int m)
     }
 }
 
+// Face node of the neighbour's trace at this side's face node m under the reference's node
+// correspondence nc_fc[ord] (constructor_face_corr, src/element/nodes_correspondence.c:87-160; the same
+// gather serves both sides, as compute_face_rlhs_T.c:133-146,312-318 applies it): LINE faces reverse
+// for ord 1; QUAD faces (m = a + N1 b) start transposed for ord >= 4, reverse a for ord 1, 3, 6, 7 and b
+// for ord 2, 3, 5, 7.
+template <int D, int P>
+__device__ inline int face_perm(int ord, int m)
+{
+    constexpr int N1 = P + 1;
+    if constexpr (D == 2) {
+        return (ord & 1) ? N1 - 1 - m : m;
+    } else {
+        const int a = m % N1, b = m / N1;
+        const int a2 = ((0xCA >> ord) & 1) ? N1 - 1 - a : a;
+        const int b2 = ((0xAC >> ord) & 1) ? N1 - 1 - b : b;
+        return ord >= 4 ? b2 + N1 * a2 : a2 + N1 * b2;
+    }
+}
+
+// Neighbour face node of face f of element e at own face node m (identity without a correspondence array).
+template <int D, int P>
+__device__ inline int nbr_node(const uint8_t* nord, int64_t e, int f, int m)
+{
+    return nord ? face_perm<D, P>(nord[e * (2 * D) + f], m) : m;
+}
+
 template <int D, int P>
 __device__ inline double face_weight(const Ops1D& op, int m)
 {
@@ -314,6 +340,7 @@ struct StageArgs {
     const double* tr_in;    // [E][NFACE][NV][NF] traces of the current state
     double* tr_out;         // [E][NFACE][NV][NF] traces of the updated state
     const uint32_t* nbr;    // [E][NFACE] packed neighbour records
+    const uint8_t* nord;    // [E][NFACE] face node correspondence case (face_perm), null when all aligned
     double* rhs_out;        // [NV][E][NS] residual output (residual mode only)
     double* blockmax;       // [grid] max |rhs| of each block (NaN propagates)
     unsigned long long* bad_elem;  // first element that failed the positivity check
@@ -466,6 +493,7 @@ struct LinArgs {
     const double* tr_u;     // [E][NFACE][NV][NF] traces of u
     const double* tr_x;     // [E][NFACE][NV][NF] traces of x
     const uint32_t* nbr;    // [E][NFACE]
+    const uint8_t* nord;    // [E][NFACE] face node correspondence case (face_perm), null when all aligned
     const int32_t* slot_elem;   // [E] element of each slot (-1 = padding) or null when every slot is used
     double* out;            // [NV][E][NS]
     int64_t n_elem;

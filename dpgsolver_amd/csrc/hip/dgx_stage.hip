@@ -414,7 +414,7 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs a)
             const int lfn = (rec >> 28) & 7;
             const bool right = (rec >> 31) != 0;
             double wn[NV], nf[NV];
-            const double* src = a.tr_in + ((en * NFACE + lfn) * NV) * NF + m;
+            const double* src = a.tr_in + ((en * NFACE + lfn) * NV) * NF + nbr_node<D, P>(a.nord, e, f, m);
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 wn[v] = src[v * NF];

@@ -777,6 +777,32 @@ static void sort_small(int* a, int n)
 
 /* Vertices of local face lf = 2*dn + side of a tensor-product element (vertex i0 + 2 i1 + 4 i2), in
  * tensor-product order over the remaining directions. */
+/* neigh_info[1].ind_ord of a face from its vertices as listed by the left volume (a) and the right volume
+ * (b), both in the reference's face-vertex order (element.c f_ve): matches_R_to_L[i] = the position in b of
+ * a[i], looked up in the table of possible matches (set_ind_ord, face.c:496-552: LINE 2 cases, QUAD 8).
+ * Returns the case, or -1 when the lists do not hold the same vertices. */
+static int face_vertex_ord(int nfv, const int* a, const int* b)
+{
+    static const int line[2][2] = { { 0, 1 }, { 1, 0 } };
+    static const int quad[8][4] = { { 0, 1, 2, 3 }, { 1, 0, 3, 2 }, { 2, 3, 0, 1 }, { 3, 2, 1, 0 },
+                                    { 0, 2, 1, 3 }, { 2, 0, 3, 1 }, { 1, 3, 0, 2 }, { 3, 1, 2, 0 } };
+    int match[4] = { -1, -1, -1, -1 };
+    for (int i = 0; i < nfv; ++i)
+        for (int j = 0; j < nfv; ++j)
+            if (b[j] == a[i])
+                match[i] = j;
+    const int n_case = (nfv == 2) ? 2 : 8;
+    for (int k = 0; k < n_case; ++k) {
+        const int* row = (nfv == 2) ? line[k] : quad[k];
+        int same = 1;
+        for (int i = 0; i < nfv; ++i)
+            same &= (match[i] == row[i]);
+        if (same)
+            return k;
+    }
+    return -1;
+}
+
 static int tp_face_vertices(int d, int lf, int* out)
 {
     const int dn = lf >> 1, side = lf & 1;
@@ -1076,13 +1102,9 @@ dgx_status dgx_mesh_read_gmsh(const char* path, int d, int64_t* n_elem, int64_t*
                 face_elem[2 * F + 1] = (int32_t)nb_e;
                 face_lf[2 * F] = (int8_t)lf;
                 face_lf[2 * F + 1] = (int8_t)nb_lf;
-                /* aligned when both volumes list the face's (substituted) vertices in the same order */
-                const int* a = &fv_sub[(e * nlf + lf) * nfv];
-                const int* b = &fv_sub[(nb_e * nlf + nb_lf) * nfv];
-                int same = 1;
-                for (int k = 0; k < nfv; ++k)
-                    same &= (a[k] == b[k]);
-                face_ord[F] = same ? 0 : 1;
+                /* neigh_info[1].ind_ord: the case of the vertex correspondence seen from the left */
+                face_ord[F] = (int8_t)face_vertex_ord(nfv, &fv_sub[(e * nlf + lf) * nfv],
+                                                      &fv_sub[(nb_e * nlf + nb_lf) * nfv]);
             }
             ++F;
         }

@@ -46,6 +46,7 @@ struct orc_ctx {
     int nthreads;
     /* explicit time stepper of orc_rk_stage: 0 SSP-RK3, 1 forward Euler, 2 low-storage RK 5-stage 4th order */
     int time_stepper;
+    int8_t* face_ord;           /* [F] neigh_info[1].ind_ord (NULL: every face aligned) */
     int64_t* ef_face;           /* [E][2d] face of each local face, built on first threaded use */
     int8_t* ef_side;            /* [E][2d] 0: the volume is the face's left volume, 1: its right */
 };
@@ -653,7 +654,7 @@ void orc_destroy(orc_ctx* c)
 {
     if (!c)
         return;
-    free(c->face_elem); free(c->face_lf);
+    free(c->face_elem); free(c->face_lf); free(c->face_ord);
     free(c->cv0_vs_vc);
     for (int j = 0; j < 3; ++j) {
         free(c->tw1_vt_vc[j]);
@@ -770,7 +771,7 @@ double orc_residual(orc_ctx* c, const double* u, double* rhs, int apply_minv)
 void orc_jvp_cs(orc_ctx* c, const double* u, const double* w, double h, double* out, int apply_minv)
 {
     const int64_t n = c->n_elem * (c->d + 2) * c->Ns;
-    double complex* uc = (double complex*)malloc(sizeof(double complex) * n);
+    double complex* uc = (double complex*)calloc((size_t)n, sizeof(double complex));
     double complex* rc = (double complex*)malloc(sizeof(double complex) * n);
     for (int64_t i = 0; i < n; ++i)
         uc[i] = u[i] + I * h * w[i];
@@ -911,6 +912,60 @@ static int rk_elem(const orc_ctx* c, int stage, double dt, int64_t e, double* u,
         }
     }
     return !c->positivity || positivity_elem(c, s, sb);
+}
+
+/* Face node correspondence nc_fc[ord] of a LINE (d_face = 1, n1 nodes) or QUAD (d_face = 2, n1 x n1 nodes)
+ * face, as constructor_face_corr builds it (src/element/nodes_correspondence.c:87-160): QUAD cases 4-7 start
+ * from the transposed ordering, cases 1, 3, 6, 7 reverse every 1-D block, cases 2, 3, 5, 7 swap the blocks
+ * end for end; a LINE reverses for case 1. */
+int orc_face_corr(int d_face, int n1, int ord, int32_t* out)
+{
+    if (d_face == 1) {
+        if (ord < 0 || ord > 1)
+            return 1;
+        for (int i = 0; i < n1; ++i)
+            out[i] = ord ? n1 - 1 - i : i;
+        return 0;
+    }
+    if (d_face != 2 || ord < 0 || ord > 7)
+        return 1;
+    const int n = n1 * n1;
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n1; ++j)
+            out[i * n1 + j] = ord >= 4 ? i + j * n1 : i * n1 + j;
+    if (ord == 1 || ord == 3 || ord == 6 || ord == 7)
+        for (int i = 0; i < n1; ++i)
+            for (int j = 0; j < n1 / 2; ++j) {
+                const int t = out[i * n1 + j];
+                out[i * n1 + j] = out[i * n1 + n1 - 1 - j];
+                out[i * n1 + n1 - 1 - j] = t;
+            }
+    if (ord == 2 || ord == 3 || ord == 5 || ord == 7)
+        for (int i = 0; i < n1 / 2; ++i)
+            for (int j = 0; j < n1; ++j) {
+                const int t = out[i * n1 + j];
+                out[i * n1 + j] = out[(n1 - 1 - i) * n1 + j];
+                out[(n1 - 1 - i) * n1 + j] = t;
+            }
+    (void)n;
+    return 0;
+}
+
+int orc_set_face_ord(orc_ctx* c, const int8_t* face_ord)
+{
+    if (c->ns && face_ord)
+        return 2;   /* the Navier-Stokes gradient path is restated for aligned faces only */
+    free(c->face_ord);
+    c->face_ord = NULL;
+    if (!face_ord)
+        return 0;
+    const int maxo = (c->d == 2) ? 1 : 7;
+    for (int64_t f = 0; f < c->n_face; ++f)
+        if (face_ord[f] < 0 || face_ord[f] > maxo)
+            return 1;
+    c->face_ord = (int8_t*)malloc((size_t)c->n_face);
+    memcpy(c->face_ord, face_ord, (size_t)c->n_face);
+    return 0;
 }
 
 int orc_set_time_stepper(orc_ctx* c, int time_stepper)

@@ -88,6 +88,15 @@ void orc_jvp_cs(orc_ctx* c, const double* u, const double* w, double h, double* 
  * first element whose average density/pressure is below EPS_PHYS. */
 int64_t orc_positivity(orc_ctx* c, double* u);
 
+/* Face node correspondence nc_fc[ord] (src/element/nodes_correspondence.c): d_face = 1 (LINE, n1 nodes,
+ * ord 0..1) or 2 (QUAD, n1 x n1 nodes, ord 0..7); out[q] = the right-volume face node seen at left face
+ * node q.  Returns 0, or 1 for an unsupported case. */
+int orc_face_corr(int d_face, int n1, int ord, int32_t* out);
+/* neigh_info[1].ind_ord of every face (NULL: all aligned): the right trace and the flux lifted into the
+ * right volume are gathered through nc_fc[ind_ord] as the reference does.  Euler only (returns 2 for a
+ * Navier-Stokes context); 1 for an out-of-range order. */
+int orc_set_face_ord(orc_ctx* c, const int8_t* face_ord);
+
 /* Explicit time stepper of orc_rk_stage (test_case->solver_type_e): 0 SSP-RK3 (default), 1 forward
  * Euler, 2 low-storage RK 5-stage 4th order (solve_explicit.c:226-350).  Returns 0, or 1 for a bad value. */
 int orc_set_time_stepper(orc_ctx* c, int time_stepper);

@@ -37,6 +37,8 @@ _SIGS = {
     "orc_positivity": (ctypes.c_int64, [ctypes.c_void_p, _dp]),
     "orc_rk_stage": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _i64p]),
     "orc_set_time_stepper": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "orc_face_corr": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p]),
+    "orc_set_face_ord": (ctypes.c_int, [ctypes.c_void_p, _i8p]),
     "orc_n_stages": (ctypes.c_int, [ctypes.c_void_p]),
     "orc_flux_point": (None, [ctypes.c_int, _dp, _dp]),
     "orc_numflux_point": (None, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
@@ -112,6 +114,13 @@ def flux_point(d, u):
     return f.reshape(d + 2, d).T  # [dim][eq]
 
 
+def face_corr(d_face, n1, ord_):
+    out = np.empty(n1 ** d_face, dtype=np.int32)
+    if load().orc_face_corr(d_face, n1, ord_, out.ctypes.data_as(_i32p)) != 0:
+        raise ValueError("unsupported face correspondence")
+    return out
+
+
 def numflux_point(d, num_flux, ul, ur, n):
     lib = load()
     ul = np.ascontiguousarray(ul, dtype=np.float64)
@@ -144,6 +153,9 @@ class Oracle:
         if viscous is not None:
             mu, pr = viscous
             lib.orc_set_viscous(self._ptr, float(mu), float(pr))
+        fo = getattr(mesh, "face_ord", None)
+        if fo is not None and np.any(fo != 0):
+            self.set_face_ord(fo)
 
     def __del__(self):
         if getattr(self, "_ptr", None):
@@ -210,6 +222,16 @@ class Oracle:
         bad = ctypes.c_int64(-1)
         mx = load().orc_rk_stage(self._ptr, stage, dt, _d(u), _d(up), ctypes.byref(bad))
         return mx, bad.value
+
+    def set_face_ord(self, face_ord):
+        """neigh_info[1].ind_ord per face (None: all aligned)."""
+        if face_ord is None:
+            rc = load().orc_set_face_ord(self._ptr, None)
+        else:
+            self._face_ord = np.ascontiguousarray(face_ord, dtype=np.int8)
+            rc = load().orc_set_face_ord(self._ptr, self._face_ord.ctypes.data_as(_i8p))
+        if rc:
+            raise ValueError(f"orc_set_face_ord: {rc}")
 
     def set_time_stepper(self, scheme):
         """0 SSP-RK3, 1 forward Euler, 2 LS-RK54 (or the names "ssp_rk_33", "euler", "ls_rk_54")."""

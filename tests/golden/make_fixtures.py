@@ -4,6 +4,8 @@ Reads (text only, nothing is executed) from the reference tree:
   input/testing/unit/nodes/nodes_tp.data          -> nodes_tp.json      (test_unit_nodes, tp)
   input/testing/unit/operators/operators_tp.data  -> operators_tp.json  (test_unit_operators_tp.c:75-176)
   input/testing/unit/containers/matrix.data       -> matrix.json        (test_unit_containers, mm/mv/mm_diag)
+  input/testing/unit/nodes/nodes_correspondence.data -> nodes_correspondence.json (test_unit_nodes,
+                                                     face_correspondence: line_3/4, quad_9/16, tri_6/10)
 The JSON files hold the numbers only (inputs and expected outputs); they are committed next to this
 script.  Run: python tests/golden/make_fixtures.py [/root/reference]
 """
@@ -135,12 +137,35 @@ def parse_matrix(path):
     return out
 
 
+def parse_correspondence(path):
+    """The Multiarray_Vector_i blocks: name -> list of index vectors (one per permutation case)."""
+    lines = _tokens(path)
+    out = {}
+    i = 0
+    while i < len(lines):
+        ln = lines[i].strip()
+        if ln.endswith(":") and i + 1 < len(lines) and lines[i + 1].strip() == "container Multiarray_Vector_i":
+            name = ln[:-1]
+            n = int(lines[i + 3].split()[1])
+            assert lines[i + 4].strip() == "ext_0/data"
+            vecs = []
+            for k in range(n):
+                t = lines[i + 5 + k].split()
+                vecs.append([int(v) for v in t[1:1 + int(t[0])]])
+            out[name] = vecs
+            i += 5 + n
+        else:
+            i += 1
+    return out
+
+
 def main(ref="/root/reference"):
     unit = os.path.join(ref, "input", "testing", "unit")
     fixtures = {
         "nodes_tp.json": parse_nodes(os.path.join(unit, "nodes", "nodes_tp.data")),
         "operators_tp.json": parse_operators(os.path.join(unit, "operators", "operators_tp.data")),
         "matrix.json": parse_matrix(os.path.join(unit, "containers", "matrix.data")),
+        "nodes_correspondence.json": parse_correspondence(os.path.join(unit, "nodes", "nodes_correspondence.data")),
     }
     for name, data in fixtures.items():
         with open(os.path.join(HERE, name), "w") as f:
