@@ -378,10 +378,6 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
         o = ops_in->line;
     }
 
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-        return DGX_ERR_NO_DEVICE;
-
     const int n1 = p + 1;
     const int ns = (d == 2) ? n1 * n1 : n1 * n1 * n1;
     const int nface = 2 * d;
@@ -416,6 +412,10 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
             return DGX_ERR_INVALID_ARG;   // boundary faces are not part of the periodic explicit path
     if (!aligned && visc)
         return DGX_ERR_INVALID_ARG;       // the Navier-Stokes kernels gather aligned faces only
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return DGX_ERR_NO_DEVICE;
 
     dgx_ctx* c = new dgx_ctx();
     c->d = d;
