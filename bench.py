@@ -460,13 +460,115 @@ def cpu_baseline_jvp(args, dg, mesh, geo, u0, p, timed):
             "gpu_jvp_per_s": gpu}
 
 
+FP64_PEAK_TFLOPS = 78.6
+MIXED_N = (60, 45, 45)
+
+
+def mixed_stage_work(mesh, tab):
+    """Algorithmic HBM bytes and FP64 flops of one stage of the batching form.
+    Bytes: per element the state read, written and the RK register read (3 x 5 Ns x 8) + metrics and
+    det J (10 x 8); per face the unit normal, face Jacobian and connectivity (5 x 8).
+    Flops (the dense per-element operator form the kernels apply): volume 2 x 5 Nc Ns (interpolation)
+    + 2 x 15 Nc Ns (weak divergence) + 3 x 22 Nc (metric-contracted fluxes); per face and side
+    2 x 5 Nq Ns (trace) + 2 x 5 Nq Ns (lift); per face node ~140 (Roe-Pike)."""
+    a = tab.arrays
+    byts = 0.0
+    flops = 0.0
+    for g in range(tab.n_group):
+        cnt = int(a["group_start"][g + 1] - a["group_start"][g])
+        ns, nc = int(a["group_ns"][g]), int(a["group_nc"][g])
+        byts += cnt * (3 * 5 * ns * 8 + 80)
+        flops += cnt * (2 * 5 * nc * ns + 2 * 15 * nc * ns + 3 * 22 * nc)
+    byts += mesh.n_face * 40
+    nq = a["fop_nq"][a["face_op"][:, 0]].astype(float)
+    nsl = a["fop_ns"][a["face_op"][:, 0]].astype(float)
+    nsr = a["fop_ns"][a["face_op"][:, 1]].astype(float)
+    flops += float((nq * (20 * (nsl + nsr) + 140)).sum())
+    return byts, flops
+
+
+def bench_mixed(args, dg, torch):
+    """Batching form: 3-D Euler on a seeded conforming mixed HEX/WEDGE/TET/PYR mesh of the unit cube
+    (about 200k elements, 40/30/20/10 % by type), per-element p uniform in 1..5, entropy wave, Roe-Pike,
+    SSP-RK3; one step = 3 stages replayed as one captured hipGraph.  Measures the autotuned persistent
+    launch (value) against the per-(type, p) launches."""
+    import numpy as np
+    from dpgsolver_amd import mixed_mesh as MM
+    from dpgsolver_amd.mixed_solver import MixedSolver, build_tables
+    t0 = time.perf_counter()
+    n = tuple(args.mixed_n)
+    mesh = MM.build_mixed_mesh(n, seed=args.seed)
+    tab = build_tables(mesh)
+    u0 = MM.project(mesh, lambda x: MM.entropy_wave(x))
+    dt = MM.dt_cfl(mesh)
+    s = MixedSolver(mesh, num_flux="roe", tables=tab)
+    s.set_state(u0)
+    print(f"[bench] mixed setup {time.perf_counter() - t0:.1f} s, E={mesh.n_elem}, dof={s.n_dof}, "
+          f"groups={tab.n_group}, dt={dt:.3e}", file=sys.stderr, flush=True)
+    default_tiles = s.get_launch()[1].tolist()
+    group_us = s.autotune(reps=3)
+    mode, tiles, bpc = s.get_launch()
+    s.set_state(u0)
+    forms = {}
+    for form in ("per_group", "persistent"):
+        s.set_launch(form)
+        us = s.time_stage(dt, 30)
+        forms[form] = {"stage_us": float(us[0]), "face_kernel_us": float(us[1]), "volume_kernel_us": float(us[2])}
+    s.set_launch("persistent")
+    if args.warmup > 0:
+        s.step(dt, args.warmup)
+    s.set_state(u0)
+    torch.cuda.synchronize()
+    tw = time.perf_counter()
+    s.step(dt, args.steps)
+    wall = time.perf_counter() - tw
+    stages = 3 * args.steps
+    # value: the timed steps (graph replays, the step call synchronizes its stream); roofline: the
+    # persistent stage time measured with HIP events on the solver's stream
+    t_stage = forms["persistent"]["stage_us"] * 1e-6
+    value = s.n_dof * stages / wall
+    byts, flops = mixed_stage_work(mesh, tab)
+    ach_bw = byts / t_stage / 1e9
+    ach_fl = flops / t_stage / 1e12
+    u = s.get_state()
+    out = {
+        "metric": "DOF-stages/s Euler mixed HEX/WEDGE/TET/PYR p=1..5 (batching form); % roofline",
+        "value": value,
+        "unit": "DOF-stages/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"3D Euler mixed mesh {n[0]}x{n[1]}x{n[2]} boxes of the periodic unit cube, "
+                                f"seed {args.seed}: " + ", ".join(f"{k} {v}" for k, v in mesh.counts().items())
+                                + "; p uniform 1..5; entropy wave; Roe-Pike; SSP-RK3 (hipGraph)"),
+                   "elements": mesh.n_elem, "faces": mesh.n_face, "dof": s.n_dof, "groups": tab.n_group, "dt": dt},
+        "roofline": {"bound": "mfma" if ach_fl / FP64_PEAK_TFLOPS > ach_bw / HBM_PEAK_GBS else "hbm",
+                     "achieved": ach_fl, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach_fl / FP64_PEAK_TFLOPS, "traffic": None,
+                     "hbm_achieved_GBs": ach_bw, "hbm_frac": ach_bw / HBM_PEAK_GBS,
+                     "algorithmic_bytes_per_stage": byts, "flops_per_stage": flops,
+                     "kernel": "face_persistent + vol_persistent", "stage_us": t_stage * 1e6},
+        "launch_forms": forms,
+        "persistent_vs_per_group": forms["per_group"]["stage_us"] / forms["persistent"]["stage_us"],
+        "autotune": {"default_tiles": default_tiles, "tiles": tiles.tolist(), "blocks_per_cu": bpc,
+                     "groups": [{"type": ["HEX", "WEDGE", "TET", "PYR"][t], "p": p_,
+                                 "elements": int(tab.arrays["group_start"][g + 1] - tab.arrays["group_start"][g]),
+                                 "volume_us": float(group_us[g])}
+                                for g, (t, p_) in enumerate(tab.group_keys)]},
+        "state_finite": bool(np.isfinite(u).all()),
+    }
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="hex64",
-                    choices=["hex64", "hex32", "vortex2d", "ns48", "vjp32", "nk", "geo"])
+                    choices=["hex64", "hex32", "vortex2d", "ns48", "vjp32", "nk", "geo", "mixed"])
+    ap.add_argument("--mixed-n", type=int, nargs=3, default=list(MIXED_N),
+                    help="boxes per direction of the mixed mesh (x layers, y, z)")
     ap.add_argument("--geo-n", type=int, default=64)
     ap.add_argument("--nk-n", type=int, default=16)
     ap.add_argument("--newton-steps", type=int, default=4)
@@ -500,6 +602,8 @@ def main():
         return bench_implicit(args, dg, torch)
     if args.config == "geo":
         return bench_geometry(args, dg, torch)
+    if args.config == "mixed":
+        return bench_mixed(args, dg, torch)
 
     t_setup = time.perf_counter()
     viscous = None

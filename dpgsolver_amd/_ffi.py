@@ -1,4 +1,4 @@
-"""ctypes declarations of the C-ABI in include/dgx.h and include/dgx_setup.h.
+"""ctypes declarations of the C-ABI in include/dgx.h, include/dgx_setup.h and include/dgx_mixed.h.
 
 The shared library ``libdgx.so`` sits next to this file (built by ``__graft_entry__.build()`` or
 ``make -C dpgsolver_amd/csrc``).  Loading fails loudly when it is missing: there is no fallback path.
@@ -63,6 +63,21 @@ class DgxOps1d(ctypes.Structure):
 
 class DgxOperators(ctypes.Structure):
     _fields_ = [("node_type", ctypes.c_int32), ("kron_order", ctypes.c_int32), ("line", DgxOps1d)]
+
+
+class DgxMixedMesh(ctypes.Structure):
+    _fields_ = [("n_elem", ctypes.c_int64), ("n_face", ctypes.c_int64), ("n_group", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("group_type", c_int32_p), ("group_p", c_int32_p),
+                ("group_ns", c_int32_p), ("group_nc", c_int32_p), ("group_start", c_int64_p),
+                ("vol_ops", c_double_p), ("vol_off", c_int64_p), ("jdet", c_double_p), ("metrics", c_double_p),
+                ("face_elem", c_int32_p), ("face_op", c_int32_p), ("face_wt", c_int32_p),
+                ("face_normal", c_double_p), ("face_jac", c_double_p), ("n_face_op", ctypes.c_int32),
+                ("n_face_wt", ctypes.c_int32), ("fop_nq", c_int32_p), ("fop_ns", c_int32_p),
+                ("fop_off", c_int64_p), ("fop_data", c_double_p), ("fw_n", c_int32_p), ("fw_off", c_int64_p),
+                ("fw_data", c_double_p)]
+
+
+MIXED_LAUNCH_PERSISTENT, MIXED_LAUNCH_PER_GROUP = 0, 1
 
 
 class DgxError(RuntimeError):
@@ -135,6 +150,21 @@ SIGNATURES = {
     "dgx_dt_cfl": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, c_double_p, ctypes.c_double,
                                      ctypes.c_double]),
     "dgx_randn": (None, [ctypes.c_uint64, ctypes.c_int64, c_double_p]),
+    # dgx_mixed.h
+    "dgx_mixed_create": (ctypes.c_int, [ctypes.POINTER(DgxMixedMesh), ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "dgx_mixed_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "dgx_mixed_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "dgx_mixed_n_dof": (ctypes.c_int64, [ctypes.c_void_p]),
+    "dgx_mixed_set_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "dgx_mixed_get_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "dgx_mixed_rhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "dgx_mixed_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]),
+    "dgx_mixed_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int32_p, ctypes.c_int32]),
+    "dgx_mixed_get_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), c_int32_p,
+                                            c_int32_p]),
+    "dgx_mixed_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p]),
+    "dgx_mixed_time_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, c_double_p]),
 }
 
 _lib = None

@@ -366,6 +366,23 @@ void orc_numflux_point(int d, int num_flux, const double* ul, const double* ur, 
         numflux_roe_d(d, ul, ur, n, nf);
 }
 
+/* Batched forms over n points (the mixed-mesh restatement evaluates every cubature point of a group at
+ * once): u [n][nv], f [n][nv][d] (the pointwise layout), ul/ur [n][nv], nrm [n][d], nf [n][nv]. */
+void orc_flux_batch(int d, int64_t n, const double* u, double* f)
+{
+    const int nv = d + 2;
+    for (int64_t i = 0; i < n; ++i)
+        flux_euler_d(d, u + i * nv, f + i * nv * d);
+}
+
+void orc_numflux_batch(int d, int num_flux, int64_t n, const double* ul, const double* ur, const double* nrm,
+                       double* nf)
+{
+    const int nv = d + 2;
+    for (int64_t i = 0; i < n; ++i)
+        orc_numflux_point(d, num_flux, ul + i * nv, ur + i * nv, nrm + i * d, nf + i * nv);
+}
+
 
 /* ------------------------------------------------------------------------------------------------ */
 /* context: operators, geometry, mass matrices                                                      */

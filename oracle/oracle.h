@@ -111,6 +111,10 @@ double orc_rk_stage(orc_ctx* c, int stage, double dt, double* u, double* u_p, in
  * f[dim + d*eq] = F_dim(eq); nf[eq]. */
 void orc_flux_point(int d, const double* u, double* f);
 void orc_numflux_point(int d, int num_flux, const double* ul, const double* ur, const double* n, double* nf);
+/* batched forms: u [n][nv] -> f [n][nv][d]; ul, ur [n][nv], nrm [n][d] -> nf [n][nv] */
+void orc_flux_batch(int d, int64_t n, const double* u, double* f);
+void orc_numflux_batch(int d, int num_flux, int64_t n, const double* ul, const double* ur, const double* nrm,
+                       double* nf);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,8 @@ _SIGS = {
     "orc_n_stages": (ctypes.c_int, [ctypes.c_void_p]),
     "orc_flux_point": (None, [ctypes.c_int, _dp, _dp]),
     "orc_numflux_point": (None, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "orc_flux_batch": (None, [ctypes.c_int, ctypes.c_int64, _dp, _dp]),
+    "orc_numflux_batch": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp]),
 }
 
 _lib = None
@@ -128,6 +130,25 @@ def numflux_point(d, num_flux, ul, ur, n):
     n = np.ascontiguousarray(n, dtype=np.float64)
     nf = np.empty(d + 2)
     lib.orc_numflux_point(d, num_flux, _d(ul), _d(ur), _d(n), _d(nf))
+    return nf
+
+
+def flux_batch(d, u):
+    """Euler fluxes of n states u [n][nv]: f [n][d][nv]."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    n = u.shape[0]
+    f = np.empty((n, d + 2, d))
+    load().orc_flux_batch(d, n, _d(u), _d(f))
+    return f.transpose(0, 2, 1)
+
+
+def numflux_batch(d, num_flux, ul, ur, nrm):
+    """Numerical fluxes n.F* of n face points: nf [n][nv]."""
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    ur = np.ascontiguousarray(ur, dtype=np.float64)
+    nrm = np.ascontiguousarray(nrm, dtype=np.float64)
+    nf = np.empty_like(ul)
+    load().orc_numflux_batch(d, num_flux, ul.shape[0], _d(ul), _d(ur), _d(nrm), _d(nf))
     return nf
 
 

@@ -2,6 +2,8 @@
 
 Reads (text only, nothing is executed) from the reference tree:
   input/testing/unit/nodes/nodes_tp.data          -> nodes_tp.json      (test_unit_nodes, tp)
+  input/testing/unit/nodes/nodes_si.data          -> nodes_si.json      (test_unit_nodes, simplex: EQ/AO/WSH/WV)
+  input/testing/unit/nodes/nodes_pyr.data         -> nodes_pyr.json     (test_unit_nodes, pyramid: GLe/GLoL/WV)
   input/testing/unit/operators/operators_tp.data  -> operators_tp.json  (test_unit_operators_tp.c:75-176)
   input/testing/unit/containers/matrix.data       -> matrix.json        (test_unit_containers, mm/mv/mm_diag)
   input/testing/unit/nodes/nodes_correspondence.data -> nodes_correspondence.json (test_unit_nodes,
@@ -163,6 +165,8 @@ def main(ref="/root/reference"):
     unit = os.path.join(ref, "input", "testing", "unit")
     fixtures = {
         "nodes_tp.json": parse_nodes(os.path.join(unit, "nodes", "nodes_tp.data")),
+        "nodes_si.json": parse_nodes(os.path.join(unit, "nodes", "nodes_si.data")),
+        "nodes_pyr.json": parse_nodes(os.path.join(unit, "nodes", "nodes_pyr.data")),
         "operators_tp.json": parse_operators(os.path.join(unit, "operators", "operators_tp.data")),
         "matrix.json": parse_matrix(os.path.join(unit, "containers", "matrix.data")),
         "nodes_correspondence.json": parse_correspondence(os.path.join(unit, "nodes", "nodes_correspondence.data")),
