@@ -69,11 +69,12 @@ class DgxMixedMesh(ctypes.Structure):
     _fields_ = [("n_elem", ctypes.c_int64), ("n_face", ctypes.c_int64), ("n_group", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("group_type", c_int32_p), ("group_p", c_int32_p),
                 ("group_ns", c_int32_p), ("group_nc", c_int32_p), ("group_start", c_int64_p),
-                ("vol_ops", c_double_p), ("vol_off", c_int64_p), ("jdet", c_double_p), ("metrics", c_double_p),
-                ("face_elem", c_int32_p), ("face_op", c_int32_p), ("face_wt", c_int32_p),
+                ("vol_ops", c_double_p), ("vol_off", c_int64_p), ("vol_tp_off", c_int64_p), ("jdet", c_double_p),
+                ("metrics", c_double_p),
+                ("face_elem", c_int32_p), ("face_lf", c_int32_p), ("face_op", c_int32_p), ("face_wt", c_int32_p),
                 ("face_normal", c_double_p), ("face_jac", c_double_p), ("n_face_op", ctypes.c_int32),
                 ("n_face_wt", ctypes.c_int32), ("fop_nq", c_int32_p), ("fop_ns", c_int32_p),
-                ("fop_off", c_int64_p), ("fop_data", c_double_p), ("fw_n", c_int32_p), ("fw_off", c_int64_p),
+                ("fop_off", c_int64_p), ("fop_red_off", c_int64_p), ("fop_data", c_double_p), ("fw_n", c_int32_p), ("fw_off", c_int64_p),
                 ("fw_data", c_double_p)]
 
 

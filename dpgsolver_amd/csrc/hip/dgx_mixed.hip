@@ -2,20 +2,23 @@
 // per-element orders, elements grouped by (type, p).
 //
 // A stage is two kernels:
-//   face    one lane per face cubature node: both sides' traces (basis rows of the side's volume at the
-//           node, the right side's node correspondence folded into its table), the numerical flux,
-//           times the cubature weight and face Jacobian -> the face flux buffer
+//   face    one lane per face cubature node: the two sides' traces from the trace buffer, the numerical
+//           flux, times the cubature weight and face Jacobian -> the face flux buffer
 //           (compute_face_rlhs_T.c:166-170, numerical_flux_euler_T.c)
 //   volume  one tile of elements of one group per workgroup: coefficients -> LDS, state at the volume
 //           cubature nodes and the metric-contracted Euler flux -> LDS, weak divergence (one lane per
 //           mode, compute_volume_rlhs_T.c:145-177), lift of the element's faces from the face flux
 //           buffer (compute_face_rlhs_T.c:297-359; the right side sees -nnf), M^-1 = 1/det J
-//           (orthonormal basis on an affine element, solve_dg.c:265-272) and the SSP-RK3 combination
-//           (solve_explicit.c:252-301)
+//           (orthonormal basis on an affine element, solve_dg.c:265-272), the SSP-RK3 combination
+//           (solve_explicit.c:252-301), then the new state's traces at the face cubature nodes of each
+//           of its faces (basis rows of the side at the nodes as the face's left volume lists them: the
+//           right side's node correspondence is folded into its table) -> the trace buffer
+// The state is read once per stage (by the volume kernel); the face kernel reads traces only.  A trace
+// pass (the volume kernel's last phase alone) fills the trace buffer after a state upload.
 // Launch forms: a persistent grid whose workgroups take tiles from a work counter (tiles of every
 // group in one launch, largest first), or one launch per (type, p) group / per face size class.  The
 // volume kernel updates its elements in place (an element reads only its own coefficients; neighbour
-// data come through the face flux buffer written by the face kernel before it).
+// data come through the trace and face flux buffers).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +41,17 @@ struct GroupDev {
     int32_t pad;
     int64_t start, count;
     int64_t op_off;        // BT [Ns][Nc] then G [3][Nc][Ns]
+    int64_t tp_off;        // HEX: B1 [n][n], D1 [n][n], w1 [n] (sum-factorized path); -1 otherwise
 };
+
+constexpr int kTpLds = 96;   // doubles at the start of a volume tile's LDS: the 1-D factors
+
+// LDS doubles of one HEX element slot: 15 n^3 (nodal fluxes) and at least 5 n^3 + 30 n^2 (mode work +
+// face-modal work of the 6 faces)
+__host__ __device__ constexpr int hex_slot(int n)
+{
+    return 15 * n * n * n > 5 * n * n * n + 30 * n * n ? 15 * n * n * n : 5 * n * n * n + 30 * n * n;
+}
 
 struct VolTile {
     int64_t e0;
@@ -54,7 +67,7 @@ struct MixArgs {
     const double* u;       // stage input
     const double* reg;     // u^n (stages 1, 2)
     double* out;           // stage output or residual
-    int stage;             // 0..2, or -1: residual
+    int stage;             // 0..2, -1: residual, -2: traces of u only
     int apply_minv;
     double dt;
     const GroupDev* groups;
@@ -63,6 +76,7 @@ struct MixArgs {
     const double* jdet;
     const double* met;     // [E][9]
     const int32_t* ef;     // [E][6]: 2 f + side, -1 = none
+    const int32_t* face_lf_of;  // [E][6]: the local face index of each entry of ef
     const int32_t* face_elem;
     const int32_t* face_op;
     const int32_t* face_wt;
@@ -70,10 +84,13 @@ struct MixArgs {
     const double* face_jac;
     const int64_t* fb_off;
     double* fb;            // face flux buffer [sum Nq][5]
+    double* tr;            // trace buffer: face f at 2 fb_off[f]: [side][Nq][5]
     const int32_t* fop_nq;
     const int32_t* fop_ns;
     const int64_t* fop_off;    // Bf [Nq][Ns]
     const int64_t* fopt_off;   // BfT [Ns][Nq]
+    const int64_t* fopr_off;   // HEX sides: face-modal rows BF [Nq][n^2] (-1: none)
+    const int64_t* foprt_off;  // and their transpose BFT [n^2][Nq]
     const double* fop;
     const int64_t* fw_off;
     const double* fw;
@@ -91,7 +108,7 @@ struct MixArgs {
 // --------------------------------------------------------------------------------------------------
 
 template <int FLUX>
-__device__ inline void face_tile(const MixArgs& a, const FaceTile& t)
+__device__ __attribute__((always_inline)) inline void face_tile(const MixArgs& a, const FaceTile& t)
 {
     const int lg = t.lg_tpf;
     const int slot = threadIdx.x >> lg, l = threadIdx.x & ((1 << lg) - 1);
@@ -102,35 +119,13 @@ __device__ inline void face_tile(const MixArgs& a, const FaceTile& t)
     const int nq = a.fop_nq[oL];
     if (l >= nq)
         return;
-    const int eL = a.face_elem[2 * f], eR = a.face_elem[2 * f + 1];
+    (void)oR;
     double wl[5], wr[5];
-    {
-        const int ns = a.fop_ns[oL];
-        const double* bt = a.fop + a.fopt_off[oL];
-        const double* ue = a.u + a.dof[eL];
+    const double* tf = a.tr + 2 * a.fb_off[f] + 5 * l;
 #pragma unroll
-        for (int v = 0; v < 5; ++v)
-            wl[v] = 0.0;
-        for (int j = 0; j < ns; ++j) {
-            const double b = bt[j * nq + l];
-#pragma unroll
-            for (int v = 0; v < 5; ++v)
-                wl[v] = fma(b, ue[v * ns + j], wl[v]);
-        }
-    }
-    {
-        const int ns = a.fop_ns[oR];
-        const double* bt = a.fop + a.fopt_off[oR];
-        const double* ue = a.u + a.dof[eR];
-#pragma unroll
-        for (int v = 0; v < 5; ++v)
-            wr[v] = 0.0;
-        for (int j = 0; j < ns; ++j) {
-            const double b = bt[j * nq + l];
-#pragma unroll
-            for (int v = 0; v < 5; ++v)
-                wr[v] = fma(b, ue[v * ns + j], wr[v]);
-        }
+    for (int v = 0; v < 5; ++v) {
+        wl[v] = tf[v];
+        wr[v] = tf[5 * nq + v];
     }
     const double n[3] = { a.face_normal[3 * f], a.face_normal[3 * f + 1], a.face_normal[3 * f + 2] };
     double nf[5];
@@ -151,17 +146,99 @@ __device__ inline double rk_mix(int stage, double dt, double s, double r, const 
     return (1.0 / 3.0) * ((*reg) + 2.0 * s + 2.0 * dt * r);
 }
 
-// One volume tile: `count` elements of group g starting at e0; every lane of the block reaches both
-// barriers.
-__device__ inline void vol_tile(const MixArgs& a, const VolTile& t, double* lds)
+// Per-mode end of the volume phase: face lifts, M^-1, the residual or the RK update (the new state also
+// goes to U, the tile's LDS copy, for the trace phase).  s_old: the stage input value of each variable.
+__device__ __attribute__((always_inline)) inline void finish_mode(const MixArgs& a, int64_t e, int ns, int j, double r[5], const double s_old[5],
+                                   double* U, bool dense_lift = true)
 {
-    const GroupDev G = a.groups[t.group];
+#pragma unroll 1
+    for (int k = 0; k < 6 && dense_lift; ++k) {
+        const int ent = a.ef[6 * e + k];
+        if (ent < 0)
+            break;
+        const int f = ent >> 1, side = ent & 1;
+        const int o = a.face_op[2 * f + side];
+        const int nq = a.fop_nq[o];
+        const double* bf = a.fop + a.fop_off[o];
+        const double* nf = a.fb + a.fb_off[f];
+        double s[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll 2
+        for (int q = 0; q < nq; ++q) {
+            const double b = bf[q * ns + j];
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                s[v] = fma(b, nf[5 * q + v], s[v]);
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+            r[v] = side ? r[v] + s[v] : r[v] - s[v];
+    }
+    double* oe = a.out + a.dof[e];
+    const double ij = a.jdet[e];
+    if (a.stage < 0) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+            oe[v * ns + j] = a.apply_minv ? r[v] / ij : r[v];
+        return;
+    }
+    const double* re = a.reg ? a.reg + a.dof[e] : nullptr;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const int i = v * ns + j;
+        const double un = rk_mix(a.stage, a.dt, s_old[v], r[v] / ij, re ? re + i : nullptr);
+        oe[i] = un;
+        U[i] = un;
+    }
+}
+
+// Traces of U at every face's cubature nodes, [side][Nq][5] at 2 fb_off[f]; the element's lanes run
+// over its face nodes one after another.
+__device__ __attribute__((always_inline)) inline void write_traces(const MixArgs& a, int64_t e, int ns, const double* U, int l, int tpe)
+{
+    int nqk[6], total = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int ent = a.ef[6 * e + k];
+        nqk[k] = ent < 0 ? 0 : a.fop_nq[a.face_op[2 * (ent >> 1) + (ent & 1)]];
+        total += nqk[k];
+    }
+    for (int it = l; it < total; it += tpe) {
+        int k = 0, q = it;
+        while (q >= nqk[k]) {
+            q -= nqk[k];
+            ++k;
+        }
+        const int ent = a.ef[6 * e + k];
+        const int nq = nqk[k];
+        const int f = ent >> 1, side = ent & 1;
+        const int o = a.face_op[2 * f + side];
+        const double* bt = a.fop + a.fopt_off[o];
+        double w[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+        // not unrolled: ns is a compile-time constant on the sum-factorized paths
+#pragma unroll 2
+        for (int j = 0; j < ns; ++j) {
+            const double b = bt[j * nq + q];
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                w[v] = fma(b, U[v * ns + j], w[v]);
+        }
+        double* t = a.tr + 2 * a.fb_off[f] + (int64_t)side * 5 * nq + 5 * q;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+            t[v] = w[v];
+    }
+}
+
+// One volume tile of a group with dense operators: `count` elements starting at e0; every lane of the
+// block reaches every barrier.
+__device__ __attribute__((always_inline)) inline void vol_tile(const MixArgs& a, const GroupDev& G, const VolTile& t, double* lds)
+{
     const int ns = G.ns, nc = G.nc;
     const int tpe = kBlock / G.tile;
     const int slot = threadIdx.x / tpe, l = threadIdx.x - slot * tpe;
     const bool act = slot < t.count;
     const int64_t e = t.e0 + slot;
-    double* U = lds + slot * (5 * ns + 15 * nc);
+    double* U = lds + kTpLds + slot * (5 * ns + 15 * nc);
     double* Fr = U + 5 * ns;
     if (act) {
         const double* ue = a.u + a.dof[e];
@@ -169,7 +246,7 @@ __device__ inline void vol_tile(const MixArgs& a, const VolTile& t, double* lds)
             U[i] = ue[i];
     }
     __syncthreads();
-    if (act) {
+    if (a.stage != -2 && act) {
         const double* bt = a.ops + G.op_off;
         double m[9];
 #pragma unroll
@@ -199,17 +276,15 @@ __device__ inline void vol_tile(const MixArgs& a, const VolTile& t, double* lds)
         }
     }
     __syncthreads();
-    if (act) {
+    if (a.stage != -2 && act) {
         const double* gm = a.ops + G.op_off + (int64_t)ns * nc;
-        const double* ue = a.u + a.dof[e];
-        double* oe = a.out + a.dof[e];
-        const double* re = a.reg ? a.reg + a.dof[e] : nullptr;
-        const double ij = a.jdet[e];
         for (int j = l; j < ns; j += tpe) {
-            double r[5];
+            double r[5], s_old[5];
 #pragma unroll
-            for (int v = 0; v < 5; ++v)
+            for (int v = 0; v < 5; ++v) {
                 r[v] = 0.0;
+                s_old[v] = U[v * ns + j];
+            }
             for (int q = 0; q < nc; ++q) {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -219,40 +294,791 @@ __device__ inline void vol_tile(const MixArgs& a, const VolTile& t, double* lds)
                         r[v] = fma(g, Fr[(q * 3 + d) * 5 + v], r[v]);
                 }
             }
-            for (int k = 0; k < 6; ++k) {
+            // no lane reads U after the barrier that closed the interpolation phase but its own modes
+            finish_mode(a, e, ns, j, r, s_old, U);
+        }
+    }
+    __syncthreads();
+    if (a.stage != -1 && act)
+        write_traces(a, e, ns, U, l, tpe);
+    __syncthreads();
+}
+
+// One volume tile of HEX p = P elements by sum factorization: the orthonormal Legendre basis at the
+// Gauss-Legendre nodes is B1 (x) B1 (x) B1 (modes and nodes r fastest), its reference gradient D1 in one
+// direction, the weights w1 (x) w1 (x) w1.  Each element works in place in 15 n^3 doubles of LDS:
+//   interpolation   three line passes (x, y, z) over the 5 variables, one lane per line
+//   fluxes          one lane per node: the 3 metric-contracted fluxes times the node weight
+//   weak divergence x: D1^T on fr_0, B1^T on fr_1, fr_2; y: B1^T x_0 + D1^T x_1, B1^T x_2;
+//                   z: B1^T y_01 + D1^T y_2  -> the volume residual in mode order
+// then the per-mode lifts / M^-1 / update and the traces as the dense path.
+template <int P>
+__device__ __attribute__((always_inline)) inline void vol_tile_hex(const MixArgs& a, const GroupDev& G, const VolTile& t, double* lds)
+{
+    constexpr int n = P + 1, n2 = n * n, n3 = n2 * n;
+    const int tpe = kBlock / G.tile;
+    const int slot = threadIdx.x / tpe, l = threadIdx.x - slot * tpe;
+    const bool act = slot < t.count;
+    const int64_t e = t.e0 + slot;
+    constexpr int per = hex_slot(n);
+    double* B1 = lds;
+    double* D1 = lds + n2;
+    double* w1 = lds + 2 * n2;
+    double* tm1 = lds + 2 * n2 + n;        // L_i(-1), L_i(+1): the traces of the 1-D basis
+    double* tp1 = lds + 2 * n2 + 2 * n;
+    double* W = lds + kTpLds + slot * per;
+    double* S = W + 5 * n3;                // face-modal work [face][v][n^2]
+    if (threadIdx.x < 2 * n2 + 3 * n)
+        lds[threadIdx.x] = a.ops[G.tp_off + threadIdx.x];
+    if (act) {
+        const double* ue = a.u + a.dof[e];
+        for (int i = l; i < 5 * n3; i += tpe)
+            W[i] = ue[i];
+    }
+    __syncthreads();
+    if (a.stage != -2) {
+        // interpolation: x, y, z line passes (stride 1, n, n^2), lines (v, two other indices)
+#pragma unroll
+        for (int pass = 0; pass < 3; ++pass) {
+            const int st = pass == 0 ? 1 : (pass == 1 ? n : n2);
+            if (act) {
+                for (int ln = l; ln < 5 * n2; ln += tpe) {
+                    const int v = ln / n2, rr = ln - v * n2;
+                    const int a0 = rr % n, a1 = rr / n;
+                    // base of the line: the two fixed indices in the other directions
+                    const int base = v * n3 + (pass == 0 ? a0 * n + a1 * n2 : (pass == 1 ? a0 + a1 * n2 : a0 + a1 * n));
+                    double in[n];
+#pragma unroll
+                    for (int i = 0; i < n; ++i)
+                        in[i] = W[base + i * st];
+#pragma unroll
+                    for (int q = 0; q < n; ++q) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int i = 0; i < n; ++i)
+                            acc = fma(B1[q * n + i], in[i], acc);
+                        W[base + q * st] = acc;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // fluxes at the nodes (a lane touches its own node only)
+        if (act) {
+            double m[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                m[i] = a.met[9 * e + i];
+            for (int q = l; q < n3; q += tpe) {
+                double w[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    w[v] = W[v * n3 + q];
+                const double wq = w1[q % n] * w1[(q / n) % n] * w1[q / n2];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double col[3] = { m[d], m[3 + d], m[6 + d] };
+                    double fd[5];
+                    dgx::euler_flux_n<3>(w, col, fd);
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        W[(d * 5 + v) * n3 + q] = wq * fd[v];
+                }
+            }
+        }
+        __syncthreads();
+        // x: lines (d, v, qy, qz)
+        if (act) {
+            for (int ln = l; ln < 15 * n2; ln += tpe) {
+                const int dv = ln / n2, rr = ln - dv * n2;
+                const double* op = dv < 5 ? D1 : B1;
+                const int base = dv * n3 + rr * n;
+                double in[n];
+#pragma unroll
+                for (int q = 0; q < n; ++q)
+                    in[q] = W[base + q];
+#pragma unroll
+                for (int i = 0; i < n; ++i) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < n; ++q)
+                        acc = fma(op[q * n + i], in[q], acc);
+                    W[base + i] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // y: lines (v, i, qz): d = 0 with d = 1 (B1^T, D1^T), and d = 2 (B1^T)
+        if (act) {
+            for (int ln = l; ln < 10 * n2; ln += tpe) {
+                const bool two = ln >= 5 * n2;
+                const int l2 = two ? ln - 5 * n2 : ln;
+                const int v = l2 / n2, rr = l2 - v * n2;
+                const int i = rr % n, kz = rr / n;
+                const int b0 = ((two ? 10 : 0) + v) * n3 + i + kz * n2;
+                const int b1 = (5 + v) * n3 + i + kz * n2;
+                double in0[n], in1[n];
+#pragma unroll
+                for (int q = 0; q < n; ++q) {
+                    in0[q] = W[b0 + q * n];
+                    in1[q] = two ? 0.0 : W[b1 + q * n];
+                }
+#pragma unroll
+                for (int j = 0; j < n; ++j) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < n; ++q) {
+                        acc = fma(B1[q * n + j], in0[q], acc);
+                        if (!two)
+                            acc = fma(D1[q * n + j], in1[q], acc);
+                    }
+                    W[b0 + j * n] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // z: lines (v, i, j): B1^T y_01 + D1^T y_2 -> residual in mode order in W[0 .. 5 n^3)
+        if (act) {
+            for (int ln = l; ln < 5 * n2; ln += tpe) {
+                const int v = ln / n2, rr = ln - v * n2;
+                const int b0 = v * n3 + rr, b2 = (10 + v) * n3 + rr;
+                double in0[n], in2[n];
+#pragma unroll
+                for (int q = 0; q < n; ++q) {
+                    in0[q] = W[b0 + q * n2];
+                    in2[q] = W[b2 + q * n2];
+                }
+#pragma unroll
+                for (int k = 0; k < n; ++k) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < n; ++q) {
+                        acc = fma(B1[q * n + k], in0[q], acc);
+                        acc = fma(D1[q * n + k], in2[q], acc);
+                    }
+                    W[b0 + k * n2] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // face lifts in face-modal form: S_k[v][ab] = sum_q BF_k[q][ab] nnf_k[q][v] (lanes over (k, ab))
+        if (act) {
+            for (int it = l; it < 6 * n2; it += tpe) {
+                const int k = it / n2, ab = it - k * n2;
                 const int ent = a.ef[6 * e + k];
-                if (ent < 0)
-                    break;
                 const int f = ent >> 1, side = ent & 1;
                 const int o = a.face_op[2 * f + side];
                 const int nq = a.fop_nq[o];
-                const double* bf = a.fop + a.fop_off[o];
+                const double* bf = a.fop + a.fopr_off[o];
                 const double* nf = a.fb + a.fb_off[f];
-                double s[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+                double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
                 for (int q = 0; q < nq; ++q) {
-                    const double b = bf[q * ns + j];
+                    const double b = bf[q * n2 + ab];
 #pragma unroll
                     for (int v = 0; v < 5; ++v)
-                        s[v] = fma(b, nf[5 * q + v], s[v]);
+                        acc[v] = fma(b, nf[5 * q + v], acc[v]);
                 }
 #pragma unroll
                 for (int v = 0; v < 5; ++v)
-                    r[v] = side ? r[v] + s[v] : r[v] - s[v];
+                    S[(k * 5 + v) * n2 + ab] = side ? acc[v] : -acc[v];
             }
-            if (a.stage < 0) {
-#pragma unroll
-                for (int v = 0; v < 5; ++v)
-                    oe[v * ns + j] = a.apply_minv ? r[v] / ij : r[v];
-            } else {
+        }
+        __syncthreads();
+        if (act) {
+            const double* ue = a.u + a.dof[e];
+            for (int j = l; j < n3; j += tpe) {
+                const int i0 = j % n, i1 = (j / n) % n, i2 = j / n2;
+                double r[5], s_old[5];
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
-                    const int i = v * ns + j;
-                    oe[i] = rk_mix(a.stage, a.dt, ue[i], r[v] / ij, re ? re + i : nullptr);
+                    r[v] = W[v * n3 + j];
+                    s_old[v] = ue[v * n3 + j];
                 }
+                // r(i0, i1, i2) += tau_lf(i_dn) S_k(the other two indices), faces in local order lf
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const int lf = a.face_lf_of[6 * e + k];
+                    const int dn = lf >> 1;
+                    const double* tau = (lf & 1) ? tp1 : tm1;
+                    const int id = dn == 0 ? i0 : (dn == 1 ? i1 : i2);
+                    const int ab = dn == 0 ? i1 + n * i2 : (dn == 1 ? i0 + n * i2 : i0 + n * i1);
+                    const double tv = tau[id];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        r[v] = fma(tv, S[(k * 5 + v) * n2 + ab], r[v]);
+                }
+                // each lane reads and then overwrites its own modes only
+                finish_mode(a, e, n3, j, r, s_old, W, false);
+            }
+        }
+        __syncthreads();
+    }
+    if (a.stage != -1) {
+        // traces in face-modal form: C_k[v][ab] = sum_i tau_lf(i) U(i along dn; ab), then
+        // trace_k(q) = sum_ab BF_k[q][ab] C_k[v][ab]
+        if (act) {
+            for (int it = l; it < 30 * n2; it += tpe) {
+                const int kv = it / n2, ab = it - kv * n2;
+                const int k = kv / 5, v = kv - 5 * k;
+                const int lf = a.face_lf_of[6 * e + k];
+                const int dn = lf >> 1;
+                const double* tau = (lf & 1) ? tp1 : tm1;
+                const int aa = ab % n, bb = ab / n;
+                const int base = v * n3 + (dn == 0 ? n * aa + n2 * bb : (dn == 1 ? aa + n2 * bb : aa + n * bb));
+                const int st = dn == 0 ? 1 : (dn == 1 ? n : n2);
+                double acc = 0.0;
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                    acc = fma(tau[i], W[base + i * st], acc);
+                S[kv * n2 + ab] = acc;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            int nqk[6], total = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int ent = a.ef[6 * e + k];
+                nqk[k] = a.fop_nq[a.face_op[2 * (ent >> 1) + (ent & 1)]];
+                total += nqk[k];
+            }
+            for (int it = l; it < total; it += tpe) {
+                int k = 0, q = it;
+                while (q >= nqk[k]) {
+                    q -= nqk[k];
+                    ++k;
+                }
+                const int ent = a.ef[6 * e + k];
+                const int nq = nqk[k];
+                const int f = ent >> 1, side = ent & 1;
+                const int o = a.face_op[2 * f + side];
+                const double* bt = a.fop + a.foprt_off[o];
+                double w[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll 4
+                for (int ab = 0; ab < n2; ++ab) {
+                    const double b = bt[ab * nq + q];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        w[v] = fma(b, S[(k * 5 + v) * n2 + ab], w[v]);
+                }
+                double* t = a.tr + 2 * a.fb_off[f] + (int64_t)side * 5 * nq + 5 * q;
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    t[v] = w[v];
             }
         }
     }
     __syncthreads();
+}
+
+// One volume tile of WEDGE p = P elements by sum factorization over TRI x LINE: the basis at the nodes
+// is B1 (x) Bt (TRI index fastest, line slowest, as the reference's WEDGE operators), its r and s
+// derivatives B1 (x) Gr, B1 (x) Gs, its t derivative D1 (x) Bt; weights w1 (x) wt.  LDS per element:
+// 15 n^3 doubles (nodal work) + 5 n Nt (modal work).
+//   interpolation   TRI (lines (v, k): Nt modes -> n^2 nodes), then LINE (lines (v, qt))
+//   fluxes          one lane per node, times the node weight
+//   weak divergence LINE^T (lines (d, v, qt): B1^T for r, s; D1^T for t), then TRI^T (lines (v, k):
+//                   Gr^T y_0 + Gs^T y_1 + Bt^T y_2 -> Nt modes)
+template <int P>
+__device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixArgs& a, const GroupDev& G, const VolTile& t, double* lds)
+{
+    constexpr int n = P + 1, n2 = n * n, n3 = n2 * n, nt = n * (n + 1) / 2, ns = n * nt;
+    const int tpe = kBlock / G.tile;
+    const int slot = threadIdx.x / tpe, l = threadIdx.x - slot * tpe;
+    const bool act = slot < t.count;
+    const int64_t e = t.e0 + slot;
+    double* B1 = lds;
+    double* D1 = lds + n2;
+    double* w1 = lds + 2 * n2;
+    const double* tri = a.ops + G.tp_off + 2 * n2 + n;   // Bt, Gr, Gs [n2][nt], wt [n2]
+    const double* Bt = tri;
+    const double* Gr = tri + n2 * nt;
+    const double* Gs = tri + 2 * n2 * nt;
+    const double* wt = tri + 3 * n2 * nt;
+    double* W = lds + kTpLds + slot * (15 * n3 + 5 * ns);
+    double* Um = W + 15 * n3;                               // modal work [v][k][m]
+    if (threadIdx.x < 2 * n2 + n)
+        lds[threadIdx.x] = a.ops[G.tp_off + threadIdx.x];
+    if (act) {
+        const double* ue = a.u + a.dof[e];
+        for (int i = l; i < 5 * ns; i += tpe)
+            Um[i] = ue[i];
+    }
+    __syncthreads();
+    if (a.stage != -2) {
+        // TRI: lines (v, k)
+        if (act) {
+            for (int it = l; it < 5 * n * n2; it += tpe) {
+                const int vk = it / n2, qt = it - vk * n2;
+                const double* in = Um + vk * nt;
+                double acc = 0.0;
+#pragma unroll 4
+                for (int m = 0; m < nt; ++m)
+                    acc = fma(Bt[qt * nt + m], in[m], acc);
+                W[vk * n2 + qt] = acc;              // [v][k][qt]
+            }
+        }
+        __syncthreads();
+        // LINE: lines (v, qt), stride n2, in place
+        if (act) {
+            for (int ln = l; ln < 5 * n2; ln += tpe) {
+                const int v = ln / n2, qt = ln - v * n2;
+                const int base = v * n3 + qt;
+                double in[n];
+#pragma unroll
+                for (int k = 0; k < n; ++k)
+                    in[k] = W[base + k * n2];
+#pragma unroll
+                for (int q = 0; q < n; ++q) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < n; ++k)
+                        acc = fma(B1[q * n + k], in[k], acc);
+                    W[base + q * n2] = acc;         // [v][ql][qt]
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {
+            double m[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                m[i] = a.met[9 * e + i];
+            for (int q = l; q < n3; q += tpe) {
+                double w[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    w[v] = W[v * n3 + q];
+                const double wq = wt[q % n2] * w1[q / n2];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double col[3] = { m[d], m[3 + d], m[6 + d] };
+                    double fd[5];
+                    dgx::euler_flux_n<3>(w, col, fd);
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        W[(d * 5 + v) * n3 + q] = wq * fd[v];
+                }
+            }
+        }
+        __syncthreads();
+        // LINE^T: lines (d, v, qt), in place
+        if (act) {
+            for (int ln = l; ln < 15 * n2; ln += tpe) {
+                const int dv = ln / n2, qt = ln - dv * n2;
+                const double* op = dv < 10 ? B1 : D1;
+                const int base = dv * n3 + qt;
+                double in[n];
+#pragma unroll
+                for (int q = 0; q < n; ++q)
+                    in[q] = W[base + q * n2];
+#pragma unroll
+                for (int k = 0; k < n; ++k) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < n; ++q)
+                        acc = fma(op[q * n + k], in[q], acc);
+                    W[base + k * n2] = acc;         // [d][v][k][qt]
+                }
+            }
+        }
+        __syncthreads();
+        // TRI^T: outputs (v, k, m) -> Um
+        if (act) {
+            for (int it = l; it < 5 * ns; it += tpe) {
+                const int vk = it / nt, mm = it - vk * nt;
+                const double* y0 = W + vk * n2;                 // d = 0: [v][k] block
+                const double* y1 = W + 5 * n3 + vk * n2;
+                const double* y2 = W + 10 * n3 + vk * n2;
+                double acc = 0.0;
+#pragma unroll 4
+                for (int qt = 0; qt < n2; ++qt) {
+                    acc = fma(Gr[qt * nt + mm], y0[qt], acc);
+                    acc = fma(Gs[qt * nt + mm], y1[qt], acc);
+                    acc = fma(Bt[qt * nt + mm], y2[qt], acc);
+                }
+                Um[it] = acc;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            const double* ue = a.u + a.dof[e];
+            // the runtime Ns (not the template constant) keeps the dense lift loops rolled
+            const int nsr = G.ns;
+            for (int j = l; j < nsr; j += tpe) {
+                double r[5], s_old[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    r[v] = Um[v * nsr + j];
+                    s_old[v] = ue[v * nsr + j];
+                }
+                finish_mode(a, e, nsr, j, r, s_old, Um);
+            }
+        }
+        __syncthreads();
+    }
+    if (a.stage != -1 && act)
+        write_traces(a, e, G.ns, Um, l, tpe);
+    __syncthreads();
+}
+
+// Collapsed-coordinate index helpers of TET / PYR order P (n = P + 1): a mode (i, j, k) exists when
+// TET: j <= P - i, k <= P - i - j; PYR: j <= P, k <= P - max(i, j); modes are numbered i slowest, k
+// fastest (the host's mode order).
+template <int P, int TYPE>
+struct Collapsed {
+    static constexpr int n = P + 1;
+    __host__ __device__ static constexpr int jmax(int i) { return TYPE == DGX_ELEM_TET ? P - i : P; }
+    __host__ __device__ static constexpr int kmax(int i, int j)
+    {
+        return TYPE == DGX_ELEM_TET ? P - i - j : P - (i > j ? i : j);
+    }
+    // first mode of the (i, j) column
+    __host__ __device__ static constexpr int off(int i, int j)
+    {
+        int o = 0;
+        for (int a = 0; a < n; ++a)
+            for (int b = 0; b <= jmax(a); ++b) {
+                if (a == i && b == j)
+                    return o;
+                o += kmax(a, b) + 1;
+            }
+        return o;
+    }
+    __host__ __device__ static constexpr int n_pairs()
+    {
+        int c = 0;
+        for (int a = 0; a < n; ++a)
+            c += jmax(a) + 1;
+        return c;
+    }
+    __host__ __device__ static constexpr int n_modes() { return off(n, 0); }
+    __host__ __device__ static constexpr int size_b() { return TYPE == DGX_ELEM_TET ? n * n * n : n * n; }
+    __host__ __device__ static constexpr int size_c() { return TYPE == DGX_ELEM_TET ? n * n * n * n : n * n * n; }
+    // (i, j) of the pair-th valid pair
+    __device__ static void pair(int pr, int& i, int& j)
+    {
+        i = 0;
+        while (pr > jmax(i)) {
+            pr -= jmax(i) + 1;
+            ++i;
+        }
+        j = pr;
+    }
+    __device__ static int bidx(int i, int q, int j) { return TYPE == DGX_ELEM_TET ? (i * n + q) * n + j : q * n + j; }
+    __device__ static int cidx(int i, int j, int r, int k)
+    {
+        return TYPE == DGX_ELEM_TET ? ((i * n + j) * n + r) * n + k : (((i > j ? i : j) * n + r) * n + k);
+    }
+};
+
+// One volume tile of TET or PYR p = P elements by sum factorization in collapsed coordinates (a, b, c):
+// the orthonormal basis is s_m times a product of one factor per coordinate (A(a), B(b), C(c); TET's
+// B depends on i and C on (i, j), PYR's C on max(i, j)), the nodes are the (a, b, c) tensor grid (a
+// fastest), and the reference gradient follows from the collapse's chain rule at each node:
+//   forward   c: lines (v, i, j) -> T1[i, j, r];  b: lines (v, i, r) -> T2[i, q, r];  a: lines (v, q, r)
+//   fluxes    per node: Ga = w (da . fr), Gb = w (db . fr), Gc = w fr_t
+//   transpose a: Ad^T Ga, A^T Gb, A^T Gc;  b: B^T y_a + Bd^T y_b, B^T y_c;  c: C^T z_ab + Cd^T z_c, times s
+// LDS per element: 15 n^3 doubles (nodal work) + 5 Ns (modal work).
+template <int P, int TYPE>
+__device__ __attribute__((always_inline)) inline void vol_tile_collapsed(const MixArgs& a, const GroupDev& G,
+                                                                        const VolTile& t, double* lds)
+{
+    using K = Collapsed<P, TYPE>;
+    constexpr int n = P + 1, n2 = n * n, n3 = n2 * n, ns = K::n_modes(), npr = K::n_pairs();
+    const int tpe = kBlock / G.tile;
+    const int slot = threadIdx.x / tpe, l = threadIdx.x - slot * tpe;
+    const bool act = slot < t.count;
+    const int64_t e = t.e0 + slot;
+    const double* tab = a.ops + G.tp_off;
+    double* Al = lds;             // A, Ad [n][n] staged in LDS
+    double* Adl = lds + n2;
+    const double* Bt = tab + 2 * n2;
+    const double* Bdt = Bt + K::size_b();
+    const double* Ct = Bdt + K::size_b();
+    const double* Cdt = Ct + K::size_c();
+    const double* xa = Cdt + K::size_c();
+    const double* xb = xa + n;
+    const double* xc = xb + n;
+    const double* wn = xc + n;
+    const double* sm = wn + n3;
+    double* W = lds + kTpLds + slot * (15 * n3 + 5 * ns);
+    double* Um = W + 15 * n3;
+    if (threadIdx.x < 2 * n2)
+        lds[threadIdx.x] = tab[threadIdx.x];
+    if (act) {
+        const double* ue = a.u + a.dof[e];
+        for (int i = l; i < 5 * ns; i += tpe)
+            Um[i] = ue[i];
+    }
+    __syncthreads();
+    if (a.stage != -2) {
+        // forward c: T1[v][i + n j + n2 r] = sum_k C(i,j)[r][k] s_m u_m
+        if (act) {
+            for (int ln = l; ln < 5 * npr; ln += tpe) {
+                const int v = ln / npr;
+                int i, j;
+                K::pair(ln - v * npr, i, j);
+                const int o = K::off(i, j), km = K::kmax(i, j);
+                double in[n];
+#pragma unroll
+                for (int k = 0; k < n; ++k)
+                    in[k] = k <= km ? sm[o + k] * Um[v * ns + o + k] : 0.0;
+#pragma unroll
+                for (int r = 0; r < n; ++r) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < n; ++k)
+                        acc = fma(Ct[K::cidx(i, j, r, k)], in[k], acc);
+                    W[v * n3 + i + n * j + n2 * r] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // forward b: lines (v, i, r), in place (j <= jmax(i) read, all q written)
+        if (act) {
+            for (int ln = l; ln < 5 * n2; ln += tpe) {
+                const int v = ln / n2, rr = ln - v * n2;
+                const int i = rr % n, r = rr / n;
+                const int base = v * n3 + i + n2 * r;
+                const int jm = K::jmax(i);
+                double in[n];
+#pragma unroll
+                for (int j = 0; j < n; ++j)
+                    in[j] = j <= jm ? W[base + n * j] : 0.0;
+#pragma unroll
+                for (int q = 0; q < n; ++q) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int j = 0; j < n; ++j)
+                        acc = fma(Bt[K::bidx(i, q, j)], in[j], acc);
+                    W[base + n * q] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // forward a: lines (v, q, r), in place
+        if (act) {
+            for (int ln = l; ln < 5 * n2; ln += tpe) {
+                const int v = ln / n2, rr = ln - v * n2;
+                const int base = v * n3 + n * rr;
+                double in[n];
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                    in[i] = W[base + i];
+#pragma unroll
+                for (int pa = 0; pa < n; ++pa) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int i = 0; i < n; ++i)
+                        acc = fma(Al[pa * n + i], in[i], acc);
+                    W[base + pa] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // fluxes at the nodes, combined through the chain rule of the collapse
+        if (act) {
+            double m[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                m[i] = a.met[9 * e + i];
+            for (int pt = l; pt < n3; pt += tpe) {
+                const int pa = pt % n, q = (pt / n) % n, r = pt / n2;
+                const double ca = xa[pa], cb = xb[q], cc = xc[r];
+                double da[3], db[3];
+                if (TYPE == DGX_ELEM_TET) {
+                    const double ib = 1.0 / ((1.0 - cb) * (1.0 - cc));
+                    da[0] = 4.0 * ib;
+                    da[1] = 2.0 * (1.0 + ca) * ib;
+                    da[2] = da[1];
+                    db[0] = 0.0;
+                    db[1] = 2.0 / (1.0 - cc);
+                    db[2] = (1.0 + cb) / (1.0 - cc);
+                } else {
+                    const double ic = 1.0 / (1.0 - cc);
+                    da[0] = 2.0 * ic;
+                    da[1] = 0.0;
+                    da[2] = ca * ic;
+                    db[0] = 0.0;
+                    db[1] = 2.0 * ic;
+                    db[2] = cb * ic;
+                }
+                double w[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    w[v] = W[v * n3 + pt];
+                const double wq = wn[pt];
+                double ga[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gb[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gc[5];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double col[3] = { m[d], m[3 + d], m[6 + d] };
+                    double fd[5];
+                    dgx::euler_flux_n<3>(w, col, fd);
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        ga[v] = fma(da[d], fd[v], ga[v]);
+                        gb[v] = fma(db[d], fd[v], gb[v]);
+                        if (d == 2)
+                            gc[v] = fd[v];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    W[v * n3 + pt] = wq * ga[v];
+                    W[(5 + v) * n3 + pt] = wq * gb[v];
+                    W[(10 + v) * n3 + pt] = wq * gc[v];
+                }
+            }
+        }
+        __syncthreads();
+        // transpose a: lines (comp, v, q, r), in place
+        if (act) {
+            for (int ln = l; ln < 15 * n2; ln += tpe) {
+                const int cv = ln / n2, rr = ln - cv * n2;
+                const double* op = cv < 5 ? Adl : Al;
+                const int base = cv * n3 + n * rr;
+                double in[n];
+#pragma unroll
+                for (int pa = 0; pa < n; ++pa)
+                    in[pa] = W[base + pa];
+#pragma unroll
+                for (int i = 0; i < n; ++i) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int pa = 0; pa < n; ++pa)
+                        acc = fma(op[pa * n + i], in[pa], acc);
+                    W[base + i] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // transpose b: lines (v, i, r): z_ab = B^T y_a + Bd^T y_b -> block 0; z_c = B^T y_c -> block 2
+        if (act) {
+            for (int ln = l; ln < 10 * n2; ln += tpe) {
+                const bool third = ln >= 5 * n2;
+                const int l2 = third ? ln - 5 * n2 : ln;
+                const int v = l2 / n2, rr = l2 - v * n2;
+                const int i = rr % n, r = rr / n;
+                const int b0 = ((third ? 10 : 0) + v) * n3 + i + n2 * r;
+                const int b1 = (5 + v) * n3 + i + n2 * r;
+                const int jm = K::jmax(i);
+                double y0[n], y1[n];
+#pragma unroll
+                for (int q = 0; q < n; ++q) {
+                    y0[q] = W[b0 + n * q];
+                    y1[q] = third ? 0.0 : W[b1 + n * q];
+                }
+#pragma unroll
+                for (int j = 0; j < n; ++j) {
+                    if (j > jm)
+                        continue;
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < n; ++q) {
+                        acc = fma(Bt[K::bidx(i, q, j)], y0[q], acc);
+                        if (!third)
+                            acc = fma(Bdt[K::bidx(i, q, j)], y1[q], acc);
+                    }
+                    W[b0 + n * j] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        // transpose c: lines (v, i, j) -> modes, times s
+        if (act) {
+            for (int ln = l; ln < 5 * npr; ln += tpe) {
+                const int v = ln / npr;
+                int i, j;
+                K::pair(ln - v * npr, i, j);
+                const int o = K::off(i, j), km = K::kmax(i, j);
+                const int b0 = v * n3 + i + n * j, b2 = (10 + v) * n3 + i + n * j;
+                double z0[n], z2[n];
+#pragma unroll
+                for (int r = 0; r < n; ++r) {
+                    z0[r] = W[b0 + n2 * r];
+                    z2[r] = W[b2 + n2 * r];
+                }
+#pragma unroll
+                for (int k = 0; k < n; ++k) {
+                    if (k > km)
+                        continue;
+                    double acc = 0.0;
+#pragma unroll
+                    for (int r = 0; r < n; ++r) {
+                        acc = fma(Ct[K::cidx(i, j, r, k)], z0[r], acc);
+                        acc = fma(Cdt[K::cidx(i, j, r, k)], z2[r], acc);
+                    }
+                    Um[v * ns + o + k] = sm[o + k] * acc;
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {
+            const double* ue = a.u + a.dof[e];
+            // the runtime Ns keeps the dense lift loops rolled
+            const int nsr = G.ns;
+            for (int j = l; j < nsr; j += tpe) {
+                double r[5], s_old[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    r[v] = Um[v * nsr + j];
+                    s_old[v] = ue[v * nsr + j];
+                }
+                finish_mode(a, e, nsr, j, r, s_old, Um);
+            }
+        }
+        __syncthreads();
+    }
+    if (a.stage != -1 && act)
+        write_traces(a, e, G.ns, Um, l, tpe);
+    __syncthreads();
+}
+
+__device__ __attribute__((always_inline)) inline void vol_dispatch(const MixArgs& a, const VolTile& t, double* lds)
+{
+    const GroupDev G = a.groups[t.group];
+    if (G.tp_off >= 0 && G.type == DGX_ELEM_TET) {
+        switch (G.p) {
+        case 1: vol_tile_collapsed<1, DGX_ELEM_TET>(a, G, t, lds); return;
+        case 2: vol_tile_collapsed<2, DGX_ELEM_TET>(a, G, t, lds); return;
+        case 3: vol_tile_collapsed<3, DGX_ELEM_TET>(a, G, t, lds); return;
+        case 4: vol_tile_collapsed<4, DGX_ELEM_TET>(a, G, t, lds); return;
+        case 5: vol_tile_collapsed<5, DGX_ELEM_TET>(a, G, t, lds); return;
+        default: break;
+        }
+    }
+    if (G.tp_off >= 0 && G.type == DGX_ELEM_PYR) {
+        switch (G.p) {
+        case 1: vol_tile_collapsed<1, DGX_ELEM_PYR>(a, G, t, lds); return;
+        case 2: vol_tile_collapsed<2, DGX_ELEM_PYR>(a, G, t, lds); return;
+        case 3: vol_tile_collapsed<3, DGX_ELEM_PYR>(a, G, t, lds); return;
+        case 4: vol_tile_collapsed<4, DGX_ELEM_PYR>(a, G, t, lds); return;
+        case 5: vol_tile_collapsed<5, DGX_ELEM_PYR>(a, G, t, lds); return;
+        default: break;
+        }
+    }
+    if (G.tp_off >= 0 && G.type == DGX_ELEM_WEDGE) {
+        switch (G.p) {
+        case 1: vol_tile_wedge<1>(a, G, t, lds); return;
+        case 2: vol_tile_wedge<2>(a, G, t, lds); return;
+        case 3: vol_tile_wedge<3>(a, G, t, lds); return;
+        case 4: vol_tile_wedge<4>(a, G, t, lds); return;
+        case 5: vol_tile_wedge<5>(a, G, t, lds); return;
+        default: break;
+        }
+    }
+    if (G.tp_off >= 0 && G.type == DGX_ELEM_HEX) {
+        switch (G.p) {
+        case 1: vol_tile_hex<1>(a, G, t, lds); return;
+        case 2: vol_tile_hex<2>(a, G, t, lds); return;
+        case 3: vol_tile_hex<3>(a, G, t, lds); return;
+        case 4: vol_tile_hex<4>(a, G, t, lds); return;
+        case 5: vol_tile_hex<5>(a, G, t, lds); return;
+        default: break;
+        }
+    }
+    vol_tile(a, G, t, lds);
 }
 
 __global__ void reset_counters(unsigned int* c)
@@ -295,14 +1121,14 @@ __global__ __launch_bounds__(kBlock) void vol_persistent(MixArgs a)
         __syncthreads();
         if (t >= a.n_vtiles)
             break;
-        vol_tile(a, a.vtiles[a.vorder[t]], lds);
+        vol_dispatch(a, a.vtiles[a.vorder[t]], lds);
     }
 }
 
 __global__ __launch_bounds__(kBlock) void vol_range(MixArgs a, int64_t t0)
 {
     extern __shared__ double lds[];
-    vol_tile(a, a.vtiles[t0 + blockIdx.x], lds);
+    vol_dispatch(a, a.vtiles[t0 + blockIdx.x], lds);
 }
 
 // --------------------------------------------------------------------------------------------------
@@ -344,15 +1170,17 @@ struct dgx_mixed_ctx {
     int n_group = 0;
     int mode = DGX_MIXED_LAUNCH_PERSISTENT;
     int bpc = 2;
+    std::vector<int32_t> tiles_of[2];    // tile shapes of each launch form
     int n_cu = 0;
     std::vector<GroupDev> groups;
     std::vector<int32_t> face_lg_of;     // per face
     // device
-    double *u = nullptr, *up = nullptr, *rhs = nullptr, *fb = nullptr;
+    double *u = nullptr, *up = nullptr, *rhs = nullptr, *fb = nullptr, *tr = nullptr;
     double *ops = nullptr, *jdet = nullptr, *met = nullptr, *fnorm = nullptr, *fjac = nullptr, *fop = nullptr,
            *fw = nullptr;
-    int64_t *dof = nullptr, *fb_off = nullptr, *fop_off = nullptr, *fopt_off = nullptr, *fw_off = nullptr;
-    int32_t *ef = nullptr, *face_elem = nullptr, *face_op = nullptr, *face_wt = nullptr, *fop_nq = nullptr,
+    int64_t *dof = nullptr, *fb_off = nullptr, *fop_off = nullptr, *fopt_off = nullptr, *fw_off = nullptr,
+            *fopr_off = nullptr, *foprt_off = nullptr;
+    int32_t *ef = nullptr, *elf = nullptr, *face_elem = nullptr, *face_op = nullptr, *face_wt = nullptr, *fop_nq = nullptr,
             *fop_ns = nullptr, *face_order = nullptr, *vorder = nullptr;
     GroupDev* d_groups = nullptr;
     VolTile* vtiles = nullptr;
@@ -386,7 +1214,25 @@ dgx_status fail(dgx_mixed_ctx* c, dgx_status s, const std::string& m)
 
 size_t tile_lds(const GroupDev& g, int tile)
 {
-    return (size_t)tile * (5 * g.ns + 15 * g.nc) * sizeof(double);
+    size_t per = 5 * (size_t)g.ns + 15 * (size_t)g.nc;      // dense: coefficients + fluxes
+    if (g.tp_off >= 0 && g.type == DGX_ELEM_HEX)
+        per = (size_t)hex_slot(g.p + 1);                     // in place, n^3 nodes = modes
+    else if (g.tp_off >= 0)
+        per = 15 * (size_t)g.nc + 5 * (size_t)g.ns;          // WEDGE: nodal + modal work
+    return (kTpLds + (size_t)tile * per) * sizeof(double);
+}
+
+// doubles of a group's factor block: B1, D1 [n][n], w1 [n]; WEDGE also Bt, Gr, Gs [n^2][Nt], wt [n^2]
+int64_t tp_len(const GroupDev& g)
+{
+    const int64_t n = g.p + 1, nt = n * (n + 1) / 2;
+    if (g.type == DGX_ELEM_TET || g.type == DGX_ELEM_PYR) {
+        // A, Ad; B, Bd; C, Cd; xa, xb, xc; node weights; mode scales
+        const int64_t sb = g.type == DGX_ELEM_TET ? n * n * n : n * n;
+        const int64_t sc = g.type == DGX_ELEM_TET ? n * n * n * n : n * n * n;
+        return 2 * n * n + 2 * sb + 2 * sc + 3 * n + n * n * n + g.ns;
+    }
+    return g.type == DGX_ELEM_WEDGE ? 2 * n * n + n + 3 * n * n * nt + n * n : 2 * n * n + 3 * n;
 }
 
 bool tile_ok(const GroupDev& g, int tile)
@@ -469,10 +1315,14 @@ MixArgs make_args(dgx_mixed_ctx* c, const double* u, const double* reg, double* 
     a.face_jac = c->fjac;
     a.fb_off = c->fb_off;
     a.fb = c->fb;
+    a.tr = c->tr;
     a.fop_nq = c->fop_nq;
     a.fop_ns = c->fop_ns;
     a.fop_off = c->fop_off;
     a.fopt_off = c->fopt_off;
+    a.fopr_off = c->fopr_off;
+    a.foprt_off = c->foprt_off;
+    a.face_lf_of = c->elf;
     a.fop = c->fop;
     a.fw_off = c->fw_off;
     a.fw = c->fw;
@@ -542,6 +1392,17 @@ hipError_t launch_eval(dgx_mixed_ctx* c, const double* u, const double* reg, dou
     if (e != hipSuccess)
         return e;
     e = launch_face(c, a, c->mode);
+    if (e != hipSuccess)
+        return e;
+    return launch_vol(c, a, c->mode);
+}
+
+// The traces of u (after a state upload): the volume kernel's trace phase alone.
+hipError_t launch_traces(dgx_mixed_ctx* c, const double* u)
+{
+    MixArgs a = make_args(c, u, nullptr, nullptr, -2, 0, 0.0);
+    reset_counters<<<1, 64, 0, c->stream>>>(c->counter);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
     return launch_vol(c, a, c->mode);
@@ -664,11 +1525,21 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
         G.start = m->group_start[g];
         G.count = m->group_start[g + 1] - G.start;
         G.op_off = m->vol_off[g];
+        G.tp_off = m->vol_tp_off ? m->vol_tp_off[g] : -1;
         MIX_CHECK(c, G.type >= 0 && G.type <= 3 && G.p >= 1 && G.p <= 5, "group type/order");
         MIX_CHECK(c, G.ns > 0 && G.ns <= 512 && G.nc > 0 && G.nc <= 512, "group Ns/Nc");
         MIX_CHECK(c, G.count > 0, "empty group");
         MIX_CHECK(c, G.op_off >= 0, "vol_off");
         vol_len = std::max<int64_t>(vol_len, G.op_off + 4LL * G.ns * G.nc);
+        if (G.tp_off >= 0) {
+            const int n = G.p + 1, nt = n * (n + 1) / 2;
+            const bool hex = G.type == DGX_ELEM_HEX && G.ns == n * n * n && G.nc == G.ns;
+            const bool wedge = G.type == DGX_ELEM_WEDGE && G.ns == n * nt && G.nc == n * n * n;
+            const bool tet = G.type == DGX_ELEM_TET && G.ns == n * (n + 1) * (n + 2) / 6 && G.nc == n * n * n;
+            const bool pyr = G.type == DGX_ELEM_PYR && G.ns == n * (n + 1) * (2 * n + 1) / 6 && G.nc == n * n * n;
+            MIX_CHECK(c, hex || wedge || tet || pyr, "factors given for a group of other sizes than its type's");
+            vol_len = std::max<int64_t>(vol_len, G.tp_off + tp_len(G));
+        }
         G.tile = default_tile(G);
         MIX_CHECK(c, tile_ok(G, G.tile), "group too large for a volume tile");
         for (int64_t e = G.start; e < G.start + G.count; ++e) {
@@ -689,6 +1560,22 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
         MIX_CHECK(c, fnq[o] > 0 && fnq[o] <= 64 && fns[o] > 0 && fns[o] <= 512 && foff[o] >= 0, "face operator size");
         fop_len = std::max<int64_t>(fop_len, foff[o] + (int64_t)fnq[o] * fns[o]);
     }
+    // face-modal rows of HEX sides: BF [Nq][n^2], n^3 = Ns
+    std::vector<int64_t> froff(m->n_face_op, -1), frofft(m->n_face_op, -1);
+    std::vector<int32_t> fn2(m->n_face_op, 0);
+    if (m->fop_red_off) {
+        for (int o = 0; o < m->n_face_op; ++o) {
+            froff[o] = m->fop_red_off[o];
+            if (froff[o] < 0)
+                continue;
+            int n = 1;
+            while (n * n * n < fns[o])
+                ++n;
+            MIX_CHECK(c, n * n * n == fns[o], "face-modal rows given for a side that is not HEX");
+            fn2[o] = n * n;
+            fop_len = std::max<int64_t>(fop_len, froff[o] + (int64_t)fnq[o] * fn2[o]);
+        }
+    }
     // transposed copies after the given data
     std::vector<double> fop(m->fop_data, m->fop_data + fop_len);
     for (int o = 0; o < m->n_face_op; ++o) {
@@ -696,6 +1583,12 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
         for (int j = 0; j < fns[o]; ++j)
             for (int q = 0; q < fnq[o]; ++q)
                 fop.push_back(m->fop_data[foff[o] + (int64_t)q * fns[o] + j]);
+        if (froff[o] >= 0) {
+            frofft[o] = (int64_t)fop.size();
+            for (int j = 0; j < fn2[o]; ++j)
+                for (int q = 0; q < fnq[o]; ++q)
+                    fop.push_back(m->fop_data[froff[o] + (int64_t)q * fn2[o] + j]);
+        }
     }
     int64_t fw_len = 0;
     for (int w = 0; w < m->n_face_wt; ++w) {
@@ -703,7 +1596,12 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
         fw_len = std::max<int64_t>(fw_len, m->fw_off[w] + m->fw_n[w]);
     }
     // faces: sides, operators, element face lists, flux buffer offsets
-    std::vector<int32_t> ef(6 * c->E, -1), nfe(c->E, 0);
+    std::vector<int32_t> ef(6 * c->E, -1), elf(6 * c->E, -1), nfe(c->E, 0);
+    std::vector<char> hex_tp(c->E, 0);
+    for (int g = 0; g < c->n_group; ++g)
+        if (c->groups[g].tp_off >= 0 && c->groups[g].type == DGX_ELEM_HEX)
+            for (int64_t e = c->groups[g].start; e < c->groups[g].start + c->groups[g].count; ++e)
+                hex_tp[e] = 1;
     std::vector<int64_t> fb_off(c->F);
     int64_t fb = 0;
     c->face_lg_of.resize(c->F);
@@ -718,7 +1616,16 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
         MIX_CHECK(c, std::isfinite(m->face_jac[f]) && m->face_jac[f] > 0.0, "face Jacobian");
         for (int s = 0; s < 2; ++s) {
             const int32_t e = s ? eR : eL;
+            const int32_t lf = m->face_lf ? m->face_lf[2 * f + s] : -1;
             MIX_CHECK(c, nfe[e] < kFacesOf[type_of[e]], "element with more faces than its type has");
+            if (hex_tp[e]) {
+                // the sum-factorized HEX path lifts and traces in face-modal form
+                MIX_CHECK(c, lf >= 0 && lf < 6 && froff[m->face_op[2 * f + s]] >= 0,
+                          "HEX sides of a sum-factorized group need face_lf and face-modal rows");
+                for (int k = 0; k < nfe[e]; ++k)
+                    MIX_CHECK(c, elf[6 * e + k] != lf, "two faces of one HEX on the same local face");
+            }
+            elf[6 * e + nfe[e]] = lf;
             ef[6 * e + nfe[e]++] = (int32_t)(2 * f + s);
         }
         fb_off[f] = fb;
@@ -760,6 +1667,8 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
                 ops[G.op_off + (int64_t)j * G.nc + q] = b[(int64_t)q * G.ns + j];
         std::memcpy(ops.data() + G.op_off + (int64_t)G.ns * G.nc, b + (int64_t)G.ns * G.nc,
                     sizeof(double) * 3 * G.ns * G.nc);
+        if (G.tp_off >= 0)
+            std::memcpy(ops.data() + G.tp_off, m->vol_ops + G.tp_off, sizeof(double) * tp_len(G));
     }
     MIX_RUN(upload(c, &c->ops, ops));
     MIX_RUN(upload(c, &c->dof, dof));
@@ -776,6 +1685,9 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     MIX_RUN(upload(c, &c->fop_ns, fns));
     MIX_RUN(upload(c, &c->fop_off, foff));
     MIX_RUN(upload(c, &c->fopt_off, fofft));
+    MIX_RUN(upload(c, &c->fopr_off, froff));
+    MIX_RUN(upload(c, &c->foprt_off, frofft));
+    MIX_RUN(upload(c, &c->elf, elf));
     MIX_RUN(upload(c, &c->fop, fop));
     MIX_RUN(upload(c, &c->fw_off, std::vector<int64_t>(m->fw_off, m->fw_off + m->n_face_wt)));
     MIX_RUN(upload(c, &c->fw, std::vector<double>(m->fw_data, m->fw_data + fw_len)));
@@ -786,9 +1698,16 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     MIX_RUN(alloc(c, &c->up, c->n_dof));
     MIX_RUN(alloc(c, &c->rhs, c->n_dof));
     MIX_RUN(alloc(c, &c->fb, c->fb_len));
+    MIX_RUN(alloc(c, &c->tr, 2 * c->fb_len));
     MIX_RUN(alloc(c, &c->counter, 2));
     MIX_TRY(c, hipMemset(c->u, 0, c->n_dof * sizeof(double)));
     MIX_TRY(c, hipMemset(c->up, 0, c->n_dof * sizeof(double)));
+    MIX_TRY(c, hipMemset(c->tr, 0, 2 * c->fb_len * sizeof(double)));
+    for (int k = 0; k < 2; ++k) {
+        c->tiles_of[k].resize(c->n_group);
+        for (int g = 0; g < c->n_group; ++g)
+            c->tiles_of[k][g] = c->groups[g].tile;
+    }
     return build_vtiles(c);
 }
 
@@ -886,7 +1805,10 @@ dgx_status dgx_mixed_set_state(dgx_mixed_ctx* ctx, int ptr_kind, const double* u
 {
     if (!ctx)
         return DGX_ERR_INVALID_ARG;
-    return copy_state(ctx, ctx->u, u, ptr_kind, hipMemcpyHostToDevice);
+    MIX_RUN(copy_state(ctx, ctx->u, u, ptr_kind, hipMemcpyHostToDevice));
+    MIX_TRY(ctx, launch_traces(ctx, ctx->u));
+    MIX_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return DGX_OK;
 }
 
 dgx_status dgx_mixed_get_state(dgx_mixed_ctx* ctx, int ptr_kind, double* u)
@@ -932,11 +1854,12 @@ dgx_status dgx_mixed_set_launch(dgx_mixed_ctx* ctx, int mode, const int32_t* til
     drop_graph(ctx);
     ctx->mode = mode;
     ctx->bpc = blocks_per_cu;
-    if (tile_elems) {
-        for (int g = 0; g < ctx->n_group; ++g)
-            ctx->groups[g].tile = tile_elems[g];
-        MIX_RUN(build_vtiles(ctx));
-    }
+    // each launch form keeps its own tile shapes (the autotuner's, or the last ones given for it)
+    if (tile_elems)
+        ctx->tiles_of[mode].assign(tile_elems, tile_elems + ctx->n_group);
+    for (int g = 0; g < ctx->n_group; ++g)
+        ctx->groups[g].tile = ctx->tiles_of[mode][g];
+    MIX_RUN(build_vtiles(ctx));
     return DGX_OK;
 }
 
@@ -961,9 +1884,12 @@ dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings)
     MIX_CHECK(ctx, reps >= 1 && reps <= 1000, "reps");
     MIX_TRY(ctx, hipStreamSynchronize(ctx->stream));
     drop_graph(ctx);
-    // the residual kernels write rhs only: the state is untouched; fill the face buffer once
+    // the residual form of the kernels writes rhs only: the state and traces are untouched; the face
+    // buffer is filled once
     MIX_TRY(ctx, launch_eval(ctx, ctx->u, nullptr, ctx->rhs, -1, 1, 0.0));
-    MixArgs a = make_args(ctx, ctx->u, nullptr, ctx->rhs, -1, 1, 0.0);
+    MixArgs a;
+    // 1. per-(type, p) launches: every group alone at every legal tile size
+    std::vector<int32_t> pg(ctx->n_group);
     for (int g = 0; g < ctx->n_group; ++g) {
         GroupDev& G = ctx->groups[g];
         int best = G.tile;
@@ -982,26 +1908,53 @@ dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings)
             }
         }
         G.tile = best;
+        pg[g] = best;
         if (timings)
             timings[g] = best_us;
     }
-    MIX_RUN(build_vtiles(ctx));
-    a = make_args(ctx, ctx->u, nullptr, ctx->rhs, -1, 1, 0.0);
+    ctx->tiles_of[DGX_MIXED_LAUNCH_PER_GROUP] = pg;
+    // 2. the persistent grid: one LDS size for every tile, so the tiles are capped to an LDS budget
+    //    (the per-group best tile, halved until it fits) and the budget and grid size are searched
+    const size_t caps[] = { 24 << 10, 32 << 10, 40 << 10, 48 << 10, (size_t)kMaxTileLds };
+    std::vector<int32_t> best_tiles = pg;
     int best_bpc = ctx->bpc;
     double best_us = 1e300;
-    for (int b = 1; b <= 4; ++b) {
-        ctx->bpc = b;
-        double us = 0.0;
-        MIX_RUN(time_launches(ctx, reps, [&]() {
-            reset_counters<<<1, 64, 0, ctx->stream>>>(ctx->counter);
-            return launch_vol(ctx, a, DGX_MIXED_LAUNCH_PERSISTENT);
-        }, &us));
-        if (us < best_us) {
-            best_us = us;
-            best_bpc = b;
+    for (size_t cap : caps) {
+        std::vector<int32_t> tl(ctx->n_group);
+        bool fits = true;
+        for (int g = 0; g < ctx->n_group && fits; ++g) {
+            int t = pg[g];
+            while (t > 1 && tile_lds(ctx->groups[g], t) > cap)
+                t /= 2;
+            fits = tile_lds(ctx->groups[g], t) <= cap;
+            tl[g] = t;
+        }
+        if (!fits)
+            continue;
+        for (int g = 0; g < ctx->n_group; ++g)
+            ctx->groups[g].tile = tl[g];
+        MIX_RUN(build_vtiles(ctx));
+        a = make_args(ctx, ctx->u, nullptr, ctx->rhs, -1, 1, 0.0);
+        const int max_bpc = std::max(1, std::min(8, (int)((160u << 10) / ctx->lds_max)));
+        for (int b = 1; b <= max_bpc; ++b) {
+            ctx->bpc = b;
+            double us = 0.0;
+            MIX_RUN(time_launches(ctx, reps, [&]() {
+                reset_counters<<<1, 64, 0, ctx->stream>>>(ctx->counter);
+                return launch_vol(ctx, a, DGX_MIXED_LAUNCH_PERSISTENT);
+            }, &us));
+            if (us < best_us) {
+                best_us = us;
+                best_bpc = b;
+                best_tiles = tl;
+            }
         }
     }
+    ctx->tiles_of[DGX_MIXED_LAUNCH_PERSISTENT] = best_tiles;
     ctx->bpc = best_bpc;
+    for (int g = 0; g < ctx->n_group; ++g)
+        ctx->groups[g].tile = ctx->tiles_of[ctx->mode][g];
+    MIX_RUN(build_vtiles(ctx));
     MIX_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return DGX_OK;
 }
@@ -1028,6 +1981,7 @@ dgx_status dgx_mixed_time_stage(dgx_mixed_ctx* ctx, double dt, int nstages, doub
         return launch_vol(ctx, a, ctx->mode);
     }, &us[2]));
     MIX_TRY(ctx, hipMemcpyAsync(ctx->u, ctx->rhs, ctx->n_dof * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    MIX_TRY(ctx, launch_traces(ctx, ctx->u));
     MIX_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return DGX_OK;
 }

@@ -334,3 +334,141 @@ def tri_operators(p):
         xc[:, d] += 1j * h
         G.append((_tri_raw(p, xc) @ R).imag / h)
     return V @ R, G[0], G[1], w
+
+
+@lru_cache(maxsize=None)
+def line_traces(p):
+    """(L_i(-1), L_i(+1)) of the orthonormal Legendre basis of order p."""
+    out = []
+    for x in (-1.0, 1.0):
+        out.append(np.array([math.sqrt((2 * k + 1) / 2.0) * _leg(k, np.array([x]))[0] for k in range(p + 1)]))
+    return out[0], out[1]
+
+
+def hex_face_modal(p, lf, Bf):
+    """The face-modal rows BF [Nq][n^2] of a HEX face operator Bf [Nq][n^3] on local face lf:
+    Bf[q][(i, a, b)] = tau_i BF[q][a + n b] with tau the 1-D basis traced to the face's side along its
+    normal direction (a, b: the other two directions, lower fastest).  Checks the factorization."""
+    n = p + 1
+    dn, side = lf >> 1, lf & 1
+    tau = line_traces(p)[side]
+    B4 = Bf.reshape(-1, n, n, n)                      # [q][k][j][i]
+    axis = {0: 3, 1: 2, 2: 1}[dn]
+    BF = np.take(B4, 0, axis=axis) / tau[0]           # [q][b][a]: the remaining axes, higher first
+    rebuilt = np.expand_dims(BF, axis) * tau.reshape([-1 if ax == axis else 1 for ax in range(4)])
+    assert np.abs(rebuilt - B4).max() < 1e-12, "face operator is not separable along the face normal"
+    return BF.reshape(BF.shape[0], n * n)
+
+
+# -------------------------------------------------------------------------------------------------------
+# collapsed-coordinate factors of TET and PYR (Sherwin & Karniadakis sec. 4.1-4.2): the raw bases are
+# orthogonal, so the orthonormal basis is the raw one times a per-mode scale s, and the raw basis is a
+# product of one function of each collapsed coordinate:
+#   TET  phi_ijk = s_ijk P_i(a) [P_j^(2i+1,0)(b) ((1-b)/2)^i] [P_k^(2i+2j+2,0)(c) ((1-c)/2)^(i+j)]
+#   PYR  phi_ijk = s_ijk P_i(a) P_j(b) [P_k^(2m+2,0)(c) ((1-c)/2)^m],  m = max(i, j)
+# and the reference gradient follows from the collapse's chain rule at each node.
+# -------------------------------------------------------------------------------------------------------
+
+def _cplx_d(f, x):
+    h = 1e-30
+    return (f(x.astype(complex) + 1j * h)).imag / h
+
+
+@lru_cache(maxsize=None)
+def collapsed_factors(etype, p):
+    """Factor tables of TET / PYR order p at the n = p+1 point rule of each collapsed direction:
+    A, Ad [n][n] (P_i(a_q), its derivative), B, Bd (TET: [i][q][j] with j <= p-i, zero-padded;
+    PYR: [q][j]), C, Cd (TET: [i][j][r][k], k <= p-i-j; PYR: [m][r][k], k <= p-m), the collapsed
+    coordinates xa, xb, xc [n], the node weights w [n^3] (volume_rule order, a fastest) and the mode scale
+    s [Ns] (mode_list order)."""
+    assert etype in (TET, PYR)
+    n = p + 1
+    xa, _ = gauss_jacobi(n, 0.0, 0.0)
+    if etype == TET:
+        xb, _ = gauss_jacobi(n, 1.0, 0.0)
+    else:
+        xb, _ = gauss_jacobi(n, 0.0, 0.0)
+    xc, _ = gauss_jacobi(n, 2.0, 0.0)
+    A = np.stack([_leg(i, xa) for i in range(n)], axis=1)
+    Ad = np.stack([_cplx_d(lambda x, i=i: _leg(i, x), xa) for i in range(n)], axis=1)
+    if etype == TET:
+        fb = lambda i, j: (lambda x: jacobi(j, 2 * i + 1.0, 0.0, x) * ((1 - x) / 2) ** i)
+        B = np.zeros((n, n, n))
+        Bd = np.zeros((n, n, n))
+        for i in range(n):
+            for j in range(n - i):
+                B[i, :, j] = fb(i, j)(xb)
+                Bd[i, :, j] = _cplx_d(fb(i, j), xb)
+        fc = lambda i, j, k: (lambda x: jacobi(k, 2 * i + 2 * j + 2.0, 0.0, x) * ((1 - x) / 2) ** (i + j))
+        C = np.zeros((n, n, n, n))
+        Cd = np.zeros((n, n, n, n))
+        for i in range(n):
+            for j in range(n - i):
+                for k in range(n - i - j):
+                    C[i, j, :, k] = fc(i, j, k)(xc)
+                    Cd[i, j, :, k] = _cplx_d(fc(i, j, k), xc)
+    else:
+        B = np.stack([_leg(j, xb) for j in range(n)], axis=1)
+        Bd = np.stack([_cplx_d(lambda x, j=j: _leg(j, x), xb) for j in range(n)], axis=1)
+        fc = lambda m, k: (lambda x: ((1 - x) / 2) ** m * jacobi(k, 2 * m + 2.0, 0.0, x))
+        C = np.zeros((n, n, n))
+        Cd = np.zeros((n, n, n))
+        for m in range(n):
+            for k in range(n - m):
+                C[m, :, k] = fc(m, k)(xc)
+                Cd[m, :, k] = _cplx_d(fc(m, k), xc)
+    _, w = volume_rule(etype, n)
+    s = np.diag(_orth(etype, p)).copy()
+    return A, Ad, B, Bd, C, Cd, xa, xb, xc, w, s
+
+
+def collapsed_dense(etype, p):
+    """The dense volume operators (B [Nc][Ns], G [3][Nc][Ns]) rebuilt from collapsed_factors by the
+    chain rule of the collapse - the kernels' sum-factorized TET / PYR paths apply exactly this."""
+    A, Ad, Bf, Bfd, Cf, Cfd, xa, xb, xc, w, s = collapsed_factors(etype, p)
+    n = p + 1
+    modes = mode_list(etype, p)
+    Bm = np.zeros((n ** 3, len(modes)))
+    Ga = np.zeros_like(Bm)
+    Gb = np.zeros_like(Bm)
+    Gc = np.zeros_like(Bm)
+    for col, (i, j, k) in enumerate(modes):
+        for r in range(n):
+            for q in range(n):
+                for pa in range(n):
+                    row = pa + n * q + n * n * r
+                    if etype == TET:
+                        fa, fad = A[pa, i], Ad[pa, i]
+                        fbv, fbd = Bf[i, q, j], Bfd[i, q, j]
+                        fcv, fcd = Cf[i, j, r, k], Cfd[i, j, r, k]
+                    else:
+                        m = max(i, j)
+                        fa, fad = A[pa, i], Ad[pa, i]
+                        fbv, fbd = Bf[q, j], Bfd[q, j]
+                        fcv, fcd = Cf[m, r, k], Cfd[m, r, k]
+                    Bm[row, col] = s[col] * fa * fbv * fcv
+                    Ga[row, col] = s[col] * fad * fbv * fcv
+                    Gb[row, col] = s[col] * fa * fbd * fcv
+                    Gc[row, col] = s[col] * fa * fbv * fcd
+    G = np.zeros((3, n ** 3, len(modes)))
+    a = np.tile(xa, n * n)
+    b = np.tile(np.repeat(xb, n), n)
+    c = np.repeat(xc, n * n)
+    for row in range(n ** 3):
+        da, db = collapse_chain(etype, a[row], b[row], c[row])
+        for d in range(3):
+            G[d, row] = da[d] * Ga[row] + db[d] * Gb[row] + (1.0 if d == 2 else 0.0) * Gc[row]
+    return Bm, G
+
+
+def collapse_chain(etype, a, b, c):
+    """(d a / d(r, s, t), d b / d(r, s, t)) of the collapse at a node (d c / d t = 1, d c / d r = d c / d s
+    = 0).  TET: r = (1+a)(1-b)(1-c)/4 - 1, s = (1+b)(1-c)/2 - 1, t = c;  PYR: r = a(1-c)/2, s = b(1-c)/2,
+    t = c."""
+    if etype == TET:
+        da = (4.0 / ((1 - b) * (1 - c)), 2.0 * (1 + a) / ((1 - b) * (1 - c)), 2.0 * (1 + a) / ((1 - b) * (1 - c)))
+        db = (0.0, 2.0 / (1 - c), (1 + b) / (1 - c))
+    else:
+        da = (2.0 / (1 - c), 0.0, a / (1 - c))
+        db = (0.0, 2.0 / (1 - c), b / (1 - c))
+    return da, db

@@ -30,7 +30,9 @@ class MixedTables:
     n_face_wt: int
 
 
-def build_tables(mesh):
+def build_tables(mesh, sum_factorized=True):
+    """sum_factorized: give every group its factor tables (the kernels' sum-factorized paths: HEX and
+    WEDGE tensor products, TET and PYR collapsed coordinates) or not (the dense path for every group)."""
     E = mesh.n_elem
     keys = [(int(t), int(p)) for t, p in zip(mesh.etype, mesh.order)]
     gkeys = sorted(set(keys))
@@ -41,7 +43,7 @@ def build_tables(mesh):
     off_mesh, ns_mesh = dof_offsets(mesh)
     gidx = np.concatenate([off_mesh[e] + np.arange(NV * ns_mesh[e]) for e in perm])
     # groups and volume operators
-    g_type, g_p, g_ns, g_nc, g_start, vol_off, vol = [], [], [], [], [0], [], []
+    g_type, g_p, g_ns, g_nc, g_start, vol_off, vol_tp_off, vol = [], [], [], [], [0], [], [], []
     pos = 0
     for (t, p) in gkeys:
         B, G, w = M.volume_operators(t, p)
@@ -54,8 +56,28 @@ def build_tables(mesh):
         blk = np.concatenate([B.ravel(), (G * w[None, :, None]).ravel()])
         vol.append(blk)
         pos += blk.size
+        if t in (M.TET, M.PYR) and sum_factorized:
+            tp = np.concatenate([x.ravel() for x in M.collapsed_factors(t, p)])
+            vol_tp_off.append(pos)
+            vol.append(tp)
+            pos += tp.size
+        elif t in (M.HEX, M.WEDGE) and sum_factorized:
+            B1, D1, w1 = M.line_operators(p)
+            tp = [B1.ravel(), D1.ravel(), w1]
+            if t == M.HEX:
+                tm, tq = M.line_traces(p)
+                tp += [tm, tq]
+            if t == M.WEDGE:
+                Bt, Gr, Gs, wt = M.tri_operators(p)
+                tp += [Bt.ravel(), Gr.ravel(), Gs.ravel(), wt]
+            tp = np.concatenate(tp)
+            vol_tp_off.append(pos)
+            vol.append(tp)
+            pos += tp.size
+        else:
+            vol_tp_off.append(-1)
     # faces
-    fop_index, fop_nq, fop_ns, fop_off, fop_data = {}, [], [], [], []
+    fop_index, fop_nq, fop_ns, fop_off, fop_red_off, fop_data = {}, [], [], [], [], []
     fw_index, fw_n, fw_off, fw_data = {}, [], [], []
     fpos = wpos = 0
     F = mesh.n_face
@@ -76,6 +98,13 @@ def build_tables(mesh):
                 fop_off.append(fpos)
                 fop_data.append(Bf.ravel())
                 fpos += Bf.size
+                if t == M.HEX and sum_factorized:
+                    BF = M.hex_face_modal(p, lf, Bf)
+                    fop_red_off.append(fpos)
+                    fop_data.append(BF.ravel())
+                    fpos += BF.size
+                else:
+                    fop_red_off.append(-1)
             face_op[f, s] = fop_index[k]
         ft = int(mesh.face_type[f])
         kw = (ft, q)
@@ -91,7 +120,7 @@ def build_tables(mesh):
         "group_type": np.array(g_type, dtype=np.int32), "group_p": np.array(g_p, dtype=np.int32),
         "group_ns": np.array(g_ns, dtype=np.int32), "group_nc": np.array(g_nc, dtype=np.int32),
         "group_start": np.array(g_start, dtype=np.int64), "vol_ops": np.concatenate(vol),
-        "vol_off": np.array(vol_off, dtype=np.int64),
+        "vol_off": np.array(vol_off, dtype=np.int64), "vol_tp_off": np.array(vol_tp_off, dtype=np.int64),
         "jdet": np.ascontiguousarray(mesh.jdet[perm], dtype=np.float64),
         "metrics": np.ascontiguousarray(mesh.metrics[perm].reshape(E, 9), dtype=np.float64),
         "face_elem": np.ascontiguousarray(inv[mesh.face_elem.astype(np.int64)], dtype=np.int32),
@@ -99,7 +128,9 @@ def build_tables(mesh):
         "face_normal": np.ascontiguousarray(mesh.normals, dtype=np.float64),
         "face_jac": np.ascontiguousarray(mesh.face_jac, dtype=np.float64),
         "fop_nq": np.array(fop_nq, dtype=np.int32), "fop_ns": np.array(fop_ns, dtype=np.int32),
-        "fop_off": np.array(fop_off, dtype=np.int64), "fop_data": np.concatenate(fop_data),
+        "fop_off": np.array(fop_off, dtype=np.int64), "fop_red_off": np.array(fop_red_off, dtype=np.int64),
+        "fop_data": np.concatenate(fop_data),
+        "face_lf": np.ascontiguousarray(mesh.face_lf, dtype=np.int32),
         "fw_n": np.array(fw_n, dtype=np.int32), "fw_off": np.array(fw_off, dtype=np.int64),
         "fw_data": np.concatenate(fw_data),
     }

@@ -23,6 +23,39 @@ SOLVER_TYPE_E = {"ssp_rk_33": _ffi.SOLVER_E_SSP_RK_33, "euler": _ffi.SOLVER_E_EU
                  "ls_rk_54": _ffi.SOLVER_E_LS_RK_54}
 
 
+EPS = 1.0e-15   # the reference's EPS (definitions_tol.h)
+
+
+def explicit_time_loop(step, dt, time_final, exit_tol=0.0, solver_proc="E", exit_ratio=0.0):
+    """The loop of solve_explicit (solve_explicit.c:92-116) and check_exit (:192-222): before each step,
+    if time + dt > time_final - 1e3 EPS then dt = time_final - time (and stays so); time += dt; one
+    step of the stepper (step(dt) returns its max_rhs); the first step's max_rhs is max_rhs0; exit when
+    max_rhs < exit_tol, and for solver_proc "E" when time == time_final, for "EI" when
+    max_rhs / max_rhs0 < exit_ratio.  Returns (time, max_rhs, steps)."""
+    if time_final < 0.0:
+        raise ValueError("time_final must be >= 0")
+    if solver_proc not in ("E", "EI"):
+        raise ValueError("solver_proc must be 'E' or 'EI'")
+    time = 0.0
+    max_rhs0 = 0.0
+    t_step = 0
+    while True:
+        if time + dt > time_final - 1e3 * EPS:
+            dt = time_final - time
+        time += dt
+        max_rhs = step(dt)
+        if t_step == 0:
+            max_rhs0 = max_rhs
+        t_step += 1
+        exit_now = max_rhs < exit_tol
+        if solver_proc == "E":
+            exit_now = exit_now or time == time_final
+        elif max_rhs0 > 0.0 and max_rhs / max_rhs0 < exit_ratio:
+            exit_now = True
+        if exit_now:
+            return time, max_rhs, t_step
+
+
 class Solver:
     """Explicit DG (Euler, QUAD/HEX, GL/GL collocated) on one GPU.
 
@@ -125,18 +158,10 @@ class Solver:
         self._check(self._lib.dgx_step(self._ctx, float(dt), int(nsteps), ctypes.byref(mx)))
         return mx.value
 
-    def solve_explicit(self, dt, time_final, exit_tol=None):
-        """Time loop of solve_explicit (solve_explicit.c:78-118): fixed dt, the last step clamped to
-        reach time_final; stops early when max_rhs < exit_tol."""
-        t = 0.0
-        mx = float("inf")
-        while t < time_final * (1 - 1e-14):
-            h = min(dt, time_final - t)
-            mx = self.step(h, 1)
-            t += h
-            if exit_tol is not None and mx < exit_tol:
-                break
-        return t, mx
+    def solve_explicit(self, dt, time_final, exit_tol=0.0, solver_proc="E", exit_ratio=0.0):
+        """Time loop of solve_explicit (solve_explicit.c:78-118) on this solver's steps; see
+        explicit_time_loop.  Returns (time, max_rhs of the last step, steps taken)."""
+        return explicit_time_loop(lambda h: self.step(h, 1), dt, time_final, exit_tol, solver_proc, exit_ratio)
 
     # -- linearization -----------------------------------------------------------------------------
     def jvp(self, w, apply_minv=False, out=None):

@@ -50,11 +50,22 @@ typedef struct dgx_mixed_mesh {
      * the reference's cv0_vs_vc and tw1_vt_vc (element_operators.c) */
     const double* vol_ops;
     const int64_t* vol_off;      /* [n_group] */
+    /* nullable; HEX and WEDGE groups: offset in vol_ops of the tensor-product factors, -1 for the dense
+     * path.  HEX: B1 [n][n] (orthonormal Legendre at the n = p+1 Gauss-Legendre nodes), D1 [n][n] (their
+     * derivatives), w1 [n] (weights), with B = B1 (x) B1 (x) B1 (r fastest) and G_d the same with D1 in
+     * direction d, then tm, tp [n] (L_i(-1), L_i(+1)).  WEDGE: B1, D1, w1, then Bt, Gr, Gs [n^2][Nt]
+     * (orthonormal TRI basis at the collapsed
+     * TRI nodes and its r, s derivatives, Nt = n(n+1)/2) and wt [n^2], with B = B1 (x) Bt (TRI fastest).
+     * The kernels then apply the volume operators by sum factorization (element_operators_tp.c:90-241
+     * builds the reference's tensor-product operators from the same factors). */
+    const int64_t* vol_tp_off;
     const double* jdet;          /* [n_elem] det of the affine map */
     const double* metrics;       /* [n_elem][3][3] m(i,j) = J d xi_j / d x_i (metrics_vc, constant) */
     /* faces: left volume face_elem[f][0], right face_elem[f][1]; outward unit normal of the left volume;
      * face Jacobian = physical face area / reference face area */
     const int32_t* face_elem;    /* [n_face][2] */
+    const int32_t* face_lf;      /* [n_face][2] local face of each side (neigh_info[s].ind_lf); nullable
+                                  * unless HEX groups are sum-factorized */
     const int32_t* face_op;      /* [n_face][2] face operator of each side */
     const int32_t* face_wt;      /* [n_face]    face weight table */
     const double* face_normal;   /* [n_face][3] */
@@ -66,6 +77,11 @@ typedef struct dgx_mixed_mesh {
     const int32_t* fop_nq;       /* [n_face_op] */
     const int32_t* fop_ns;       /* [n_face_op] */
     const int64_t* fop_off;      /* [n_face_op] */
+    /* nullable; HEX sides: offset in fop_data of the face-modal rows BF [Nq][n^2] with
+     * B_f[q][(i, a, b)] = tau_i BF[q][a + n b], tau = L_i(-1) or L_i(+1) along the face's normal
+     * direction and (a, b) the other two directions, lower one fastest (required for the HEX sides of
+     * sum-factorized groups: their lifts and traces contract the face-modal form); -1 otherwise */
+    const int64_t* fop_red_off;
     const double* fop_data;
     const int32_t* fw_n;         /* [n_face_wt] face cubature nodes */
     const int64_t* fw_off;       /* [n_face_wt] */
@@ -96,15 +112,19 @@ dgx_status dgx_mixed_rhs(dgx_mixed_ctx* ctx, int apply_minv, int ptr_kind, doubl
 dgx_status dgx_mixed_step(dgx_mixed_ctx* ctx, double dt, int nsteps);
 
 /* Launch form and tile shapes.  tile_elems[g]: elements per workgroup tile of group g (a power of two,
- * 1..64, with the group's lanes per element = 256 / tile_elems[g] >= 4); NULL keeps the current ones.
+ * 1..64, with the group's lanes per element = 256 / tile_elems[g] >= 4); each launch form keeps its own
+ * tile shapes, and NULL selects the ones it has (the autotuner's, or the last given for it).
  * blocks_per_cu: workgroups per CU of the persistent grid (1..8).  Drops the captured graph. */
 dgx_status dgx_mixed_set_launch(dgx_mixed_ctx* ctx, int mode, const int32_t* tile_elems, int32_t blocks_per_cu);
 dgx_status dgx_mixed_get_launch(const dgx_mixed_ctx* ctx, int* mode, int32_t* tile_elems, int32_t* blocks_per_cu);
 
-/* The autotuner: for every group, times the volume stage kernel of that group alone at each legal
- * tile_elems (reps launches each, HIP events on the context's stream) and keeps the fastest; then
- * times the persistent grid at blocks_per_cu 1..4 and keeps the fastest.  The state is restored.
- * timings (nullable) receives per group the best microseconds per launch. */
+/* The autotuner (both launch forms; the current form is kept): for every group, times the volume kernel
+ * of that group alone at each legal tile_elems (reps launches each, HIP events on the context's stream)
+ * and keeps the fastest for the per-group form; for the persistent form, whose one launch has one LDS
+ * size, caps those tiles to an LDS budget (24..60 KB, halving a tile until it fits) and times the whole
+ * persistent volume launch for every budget and every blocks_per_cu the budget lets fit on a CU,
+ * keeping the fastest.  Runs the residual form of the kernels: the state is untouched.  timings
+ * (nullable) receives per group the best microseconds per launch. */
 dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings);
 
 /* Mean microseconds per stage of nstages SSP-RK3 stages (graph replays, events on the context's

@@ -156,3 +156,36 @@ def test_failure_stops_multi_step_call(kind, scheme):
     a.step(dt, 1)
     a.close()
     b.close()
+
+
+def test_solve_explicit_loop_vs_oracle():
+    """Solver.solve_explicit (the reference's loop: clamp of the last step by time + dt > time_final -
+    1e3 EPS, exit on time == time_final) equals the same loop over the oracle's steps: the same step
+    sizes (the last one a fraction of dt) and the same final state; and an exit tolerance above the first
+    step's max_rhs stops after one step on both."""
+    _require_gpu()
+    print("case: solve_explicit loop vs oracle loop", flush=True)
+    from dpgsolver_amd.solver import explicit_time_loop
+    mesh, geo, p, u0, visc, cfl = _case("quad2")
+    dt = cfl * dg.dt_cfl(mesh, p, u0)
+    tf = 3.4 * dt
+    orc = Oracle(mesh, p, num_flux=0, geometry=geo)
+    u_ref, up = u0.copy(), np.zeros_like(u0)
+    dts = []
+
+    def orc_step(h):
+        dts.append(h)
+        return orc.step(h, u_ref, up, 1)
+
+    t_ref, mx_ref, n_ref = explicit_time_loop(orc_step, dt, tf)
+    assert n_ref == 4 and t_ref == tf and 0 < dts[-1] < dt
+    s = dg.Solver(mesh, geo, p, num_flux="lf")
+    s.set_state(u0)
+    t, mx, n = s.solve_explicit(dt, tf)
+    assert (t, n) == (t_ref, n_ref)
+    assert abs(mx - mx_ref) <= 1e-11 * abs(mx_ref)
+    assert rel_err(s.get_state(), u_ref) < TOL
+    s.set_state(u0)
+    t, mx, n = s.solve_explicit(dt, tf, exit_tol=10 * mx_ref + 1e3)
+    assert n == 1 and t == dt
+    s.close()

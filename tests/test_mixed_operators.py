@@ -194,3 +194,16 @@ def test_tensor_product_factors(p):
     assert np.abs(np.kron(B1, Gs) - G[1]).max() < 1e-10
     assert np.abs(np.kron(D1, Bt) - G[2]).max() < 1e-10
     assert np.abs(np.kron(w1, wt) - w).max() < 1e-15
+
+
+@pytest.mark.parametrize("etype", [M.TET, M.PYR])
+@pytest.mark.parametrize("p", [1, 2, 3, 5])
+def test_collapsed_factors(etype, p):
+    """The TET / PYR volume operators are the scaled products of one factor per collapsed coordinate,
+    and their reference gradients follow from the collapse's chain rule: the tables the sum-factorized
+    kernels contract rebuild the dense operators."""
+    B, G, w = M.volume_operators(etype, p)
+    Bc, Gc = M.collapsed_dense(etype, p)
+    assert np.abs(Bc - B).max() < 1e-12
+    assert np.abs(Gc - G).max() < 1e-10 * max(1.0, np.abs(G).max())
+    assert np.abs(M.collapsed_factors(etype, p)[9] - w).max() < 1e-15
