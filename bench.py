@@ -465,12 +465,12 @@ MIXED_N = (60, 45, 45)
 
 
 def mixed_stage_work(mesh, tab):
-    """Algorithmic HBM bytes and FP64 flops of one stage of the batching form.
+    """Algorithmic HBM bytes and a flop scale of one stage of the batching form.
     Bytes: per element the state read, written and the RK register read (3 x 5 Ns x 8) + metrics and
     det J (10 x 8); per face the unit normal, face Jacobian and connectivity (5 x 8).
-    Flops (the dense per-element operator form the kernels apply): volume 2 x 5 Nc Ns (interpolation)
-    + 2 x 15 Nc Ns (weak divergence) + 3 x 22 Nc (metric-contracted fluxes); per face and side
-    2 x 5 Nq Ns (trace) + 2 x 5 Nq Ns (lift); per face node ~140 (Roe-Pike)."""
+    Flops, for scale only (the dense per-element operator form; the sum-factorized kernels execute fewer):
+    volume 2 x 5 Nc Ns (interpolation) + 2 x 15 Nc Ns (weak divergence) + 3 x 22 Nc (fluxes); per face
+    and side 2 x 5 Nq Ns (trace) + 2 x 5 Nq Ns (lift); per face node ~140 (Roe-Pike)."""
     a = tab.arrays
     byts = 0.0
     flops = 0.0
@@ -541,12 +541,14 @@ def bench_mixed(args, dg, torch):
                                 f"seed {args.seed}: " + ", ".join(f"{k} {v}" for k, v in mesh.counts().items())
                                 + "; p uniform 1..5; entropy wave; Roe-Pike; SSP-RK3 (hipGraph)"),
                    "elements": mesh.n_elem, "faces": mesh.n_face, "dof": s.n_dof, "groups": tab.n_group, "dt": dt},
-        "roofline": {"bound": "mfma" if ach_fl / FP64_PEAK_TFLOPS > ach_bw / HBM_PEAK_GBS else "hbm",
-                     "achieved": ach_fl, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach_fl / FP64_PEAK_TFLOPS, "traffic": None,
-                     "hbm_achieved_GBs": ach_bw, "hbm_frac": ach_bw / HBM_PEAK_GBS,
-                     "algorithmic_bytes_per_stage": byts, "flops_per_stage": flops,
-                     "kernel": "face_persistent + vol_persistent", "stage_us": t_stage * 1e6},
+        "roofline": {"bound": "hbm", "achieved": ach_bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach_bw / HBM_PEAK_GBS,
+                     "traffic": read_traffic(os.path.join(HERE, "profiles", "stage_mixed_200k_traffic.md")),
+                     "algorithmic_bytes_per_stage": byts, "stage_us": t_stage * 1e6,
+                     "kernel": "vol_persistent + face_persistent (+ reset_counters)",
+                     "fp64_dense_equivalent_flops_per_stage": flops,
+                     "fp64_dense_equivalent_TFLOPs": ach_fl, "fp64_peak_TFLOPs": FP64_PEAK_TFLOPS},
+        "persistent_waves_per_simd": s.persistent_waves(),
         "launch_forms": forms,
         "persistent_vs_per_group": forms["per_group"]["stage_us"] / forms["persistent"]["stage_us"],
         "autotune": {"default_tiles": default_tiles, "tiles": tiles.tolist(), "blocks_per_cu": bpc,
@@ -557,7 +559,34 @@ def bench_mixed(args, dg, torch):
         "state_finite": bool(np.isfinite(u).all()),
     }
     s.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_mixed(args)
     print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_mixed(args):
+    """The mixed-mesh oracle (numpy over the oracle library's flux functions, one thread) on a bounded
+    sample of the same construction: residual evaluations (the stage's work) for about cpu_seconds."""
+    import numpy as np
+    from dpgsolver_amd import mixed_mesh as MM
+    from oracle.mixed_oracle import MixedOracle
+    n = (8, 4, 4)
+    mesh = MM.build_mixed_mesh(n, seed=args.seed)
+    o = MixedOracle(mesh, num_flux=1)
+    u = MM.project(mesh, lambda x: MM.entropy_wave(x))
+    o.residual(u)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        o.residual(u)
+        k += 1
+        el = time.perf_counter() - t0
+        if el > args.cpu_seconds:
+            break
+    return {"value": o.n_dof * k / el, "unit": "DOF-stages/s", "cores": 1, "kind": "port",
+            "sample": (f"mixed mesh {n[0]}x{n[1]}x{n[2]} boxes ({mesh.n_elem} elements, {o.n_dof} DOF, "
+                       f"p 1..5), {k} residual evaluations in {el:.1f} s"),
+            "cpu": cpu_model()}
 
 
 def main():

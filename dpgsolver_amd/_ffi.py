@@ -74,7 +74,7 @@ class DgxMixedMesh(ctypes.Structure):
                 ("face_elem", c_int32_p), ("face_lf", c_int32_p), ("face_op", c_int32_p), ("face_wt", c_int32_p),
                 ("face_normal", c_double_p), ("face_jac", c_double_p), ("n_face_op", ctypes.c_int32),
                 ("n_face_wt", ctypes.c_int32), ("fop_nq", c_int32_p), ("fop_ns", c_int32_p),
-                ("fop_off", c_int64_p), ("fop_red_off", c_int64_p), ("fop_data", c_double_p), ("fw_n", c_int32_p), ("fw_off", c_int64_p),
+                ("fop_off", c_int64_p), ("fop_red_off", c_int64_p), ("fop_red_n", c_int32_p), ("fop_data", c_double_p), ("fw_n", c_int32_p), ("fw_off", c_int64_p),
                 ("fw_data", c_double_p)]
 
 
@@ -165,6 +165,7 @@ SIGNATURES = {
     "dgx_mixed_get_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), c_int32_p,
                                             c_int32_p]),
     "dgx_mixed_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p]),
+    "dgx_mixed_persistent_waves": (ctypes.c_int, [ctypes.c_void_p]),
     "dgx_mixed_time_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, c_double_p]),
 }
 

@@ -77,7 +77,7 @@ __device__ __forceinline__ void panel(float (&a)[R][R], unsigned& used, int& fla
         const float v = __uint_as_float(khi);
         int p = (int)~klo;
         if (!(v > 0.f))
-            flag = k + 1;       // no usable pivot (singular block or non-finite input)
+            flag = k + 1;       // no usable pivot (singular block; non-finite values are voted apart)
         p = (p < n) ? p : k;    // keeps every index in range when the block is singular
         const int rp = p >> 5;
         const bool prow_lane = ti == (p & 31);
@@ -228,6 +228,7 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const
     __shared__ int s_cidx[2][TG];
     __shared__ int s_piv[NP];
     __shared__ int s_step[NP];
+    __shared__ int s_bad;                      // a non-finite entry in the block or its inverse
 
     const int tj = threadIdx.x & (TG - 1);
     const int ti = threadIdx.x >> 5;
@@ -249,6 +250,15 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const
         s_piv[threadIdx.x] = 0;
         s_step[threadIdx.x] = 0;
     }
+    if (threadIdx.x == 0)
+        s_bad = 0;
+    // non-finite input: NaN never wins a pivot search and would otherwise pass unflagged
+    bool bad = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int c = 0; c < R; ++c)
+            bad |= !isfinite(a[r][c]);
     unsigned used = 0;              // bit r: row ti + 32 r has been a pivot row
     int flag = 0;
 
@@ -271,6 +281,14 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const
 
     const InvShared sm{ s_w, s_slab, s_prow, &s_pcol[0][0], s_cval, s_cidx, s_piv, s_step };
     Panels<R, 0>::run(a, used, flag, n, ti, tj, sm);
+    // non-finite inverse: a denormal pivot overflows its reciprocal
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int c = 0; c < R; ++c)
+            bad |= !isfinite(a[r][c]);
+    if (bad)
+        atomicOr(&s_bad, 1);
     __syncthreads();
 
     float* ob = out + blk * (int64_t)n * n;
@@ -288,7 +306,7 @@ __global__ __launch_bounds__(1024) void block_inverse_kernel(const TIN* B, const
         }
     }
     if (threadIdx.x == 0)
-        singular[blk] = flag;
+        singular[blk] = flag ? flag : (s_bad ? n + 1 : 0);
 }
 
 // y_b = A_b x_b for n x n FP32 blocks stored column-major (A_b[j][i] at col[b][j * n + i]), FP64 vectors:

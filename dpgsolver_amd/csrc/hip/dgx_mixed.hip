@@ -91,6 +91,7 @@ struct MixArgs {
     const int64_t* fopt_off;   // BfT [Ns][Nq]
     const int64_t* fopr_off;   // HEX sides: face-modal rows BF [Nq][n^2] (-1: none)
     const int64_t* foprt_off;  // and their transpose BFT [n^2][Nq]
+    const int32_t* fopr_n;     // columns of the face-modal rows (HEX, WEDGE quad: n^2; WEDGE TRI: Nt)
     const double* fop;
     const int64_t* fw_off;
     const double* fw;
@@ -595,8 +596,12 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
     const double* Gr = tri + n2 * nt;
     const double* Gs = tri + 2 * n2 * nt;
     const double* wt = tri + 3 * n2 * nt;
+    const double* Ew = wt + n2;                             // E [3][n][Nt]: the TRI edge matrices
+    const double* tmw = Ew + 3 * n * nt;                    // L_k(-1), L_k(+1)
+    const double* tpw = tmw + n;
     double* W = lds + kTpLds + slot * (15 * n3 + 5 * ns);
     double* Um = W + 15 * n3;                               // modal work [v][k][m]
+    double* S = W;                                          // face-modal work [face][v][n^2] (25 n^2 <= 15 n^3)
     if (threadIdx.x < 2 * n2 + n)
         lds[threadIdx.x] = a.ops[G.tp_off + threadIdx.x];
     if (act) {
@@ -701,30 +706,132 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
             }
         }
         __syncthreads();
+        // face lifts in face-modal form (the nodal work area is free): S_k[v][x] = sum_q BF_k[q][x] nnf[q][v]
+        if (act) {
+            for (int it = l; it < 5 * n2; it += tpe) {
+                const int k = it / n2, x = it - k * n2;
+                const int ent = a.ef[6 * e + k];
+                const int f = ent >> 1, side = ent & 1;
+                const int o = a.face_op[2 * f + side];
+                const int wdt = a.fopr_n[o];
+                if (x >= wdt)
+                    continue;
+                const int nq = a.fop_nq[o];
+                const double* bf = a.fop + a.fopr_off[o];
+                const double* nf = a.fb + a.fb_off[f];
+                double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+                for (int q = 0; q < nq; ++q) {
+                    const double b = bf[q * wdt + x];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        acc[v] = fma(b, nf[5 * q + v], acc[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    S[(k * 5 + v) * n2 + x] = side ? acc[v] : -acc[v];
+            }
+        }
+        __syncthreads();
         if (act) {
             const double* ue = a.u + a.dof[e];
-            // the runtime Ns (not the template constant) keeps the dense lift loops rolled
-            const int nsr = G.ns;
-            for (int j = l; j < nsr; j += tpe) {
+            for (int j = l; j < ns; j += tpe) {
+                const int kl = j / nt, mm = j - kl * nt;
                 double r[5], s_old[5];
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
-                    r[v] = Um[v * nsr + j];
-                    s_old[v] = ue[v * nsr + j];
+                    r[v] = Um[v * ns + j];
+                    s_old[v] = ue[v * ns + j];
                 }
-                finish_mode(a, e, nsr, j, r, s_old, Um);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    const int lf = a.face_lf_of[6 * e + k];
+                    if (lf < 3) {
+                        // quad face over TRI edge lf: sum_a E[a][m] S[a + n kl]
+                        const double* E = Ew + lf * n * nt;
+#pragma unroll
+                        for (int aa = 0; aa < n; ++aa) {
+                            const double ev = E[aa * nt + mm];
+#pragma unroll
+                            for (int v = 0; v < 5; ++v)
+                                r[v] = fma(ev, S[(k * 5 + v) * n2 + aa + n * kl], r[v]);
+                        }
+                    } else {
+                        const double tv = (lf == 3 ? tmw : tpw)[kl];
+#pragma unroll
+                        for (int v = 0; v < 5; ++v)
+                            r[v] = fma(tv, S[(k * 5 + v) * n2 + mm], r[v]);
+                    }
+                }
+                // each lane reads and then overwrites its own modes only
+                finish_mode(a, e, ns, j, r, s_old, Um, false);
             }
         }
         __syncthreads();
     }
-    if (a.stage != -1 && act)
-        write_traces(a, e, G.ns, Um, l, tpe);
+    if (a.stage != -1) {
+        // traces in face-modal form: quad faces c[a + n k] = sum_m E[a][m] U[k][m], TRI faces
+        // c[m] = sum_k tau_k U[k][m]; then trace(q) = sum_x BF[q][x] c[x]
+        if (act) {
+            for (int it = l; it < 25 * n2; it += tpe) {
+                const int kv = it / n2, x = it - kv * n2;
+                const int k = kv / 5, v = kv - 5 * k;
+                const int lf = a.face_lf_of[6 * e + k];
+                double acc = 0.0;
+                if (lf < 3) {
+                    const int aa = x % n, kl = x / n;
+                    const double* E = Ew + lf * n * nt + aa * nt;
+                    const double* u = Um + v * ns + kl * nt;
+                    for (int mm = 0; mm < nt; ++mm)
+                        acc = fma(E[mm], u[mm], acc);
+                } else {
+                    if (x >= nt)
+                        continue;
+                    const double* tau = lf == 3 ? tmw : tpw;
+#pragma unroll
+                    for (int kl = 0; kl < n; ++kl)
+                        acc = fma(tau[kl], Um[v * ns + kl * nt + x], acc);
+                }
+                S[kv * n2 + x] = acc;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            int nqk[5], total = 0;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const int ent = a.ef[6 * e + k];
+                nqk[k] = a.fop_nq[a.face_op[2 * (ent >> 1) + (ent & 1)]];
+                total += nqk[k];
+            }
+            for (int it = l; it < total; it += tpe) {
+                int k = 0, q = it;
+                while (q >= nqk[k]) {
+                    q -= nqk[k];
+                    ++k;
+                }
+                const int ent = a.ef[6 * e + k];
+                const int nq = nqk[k];
+                const int f = ent >> 1, side = ent & 1;
+                const int o = a.face_op[2 * f + side];
+                const int wdt = a.fopr_n[o];
+                const double* bt = a.fop + a.foprt_off[o];
+                double w[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+                for (int x = 0; x < wdt; ++x) {
+                    const double b = bt[x * nq + q];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        w[v] = fma(b, S[(k * 5 + v) * n2 + x], w[v]);
+                }
+                double* t = a.tr + 2 * a.fb_off[f] + (int64_t)side * 5 * nq + 5 * q;
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    t[v] = w[v];
+            }
+        }
+    }
     __syncthreads();
 }
 
-// Collapsed-coordinate index helpers of TET / PYR order P (n = P + 1): a mode (i, j, k) exists when
-// TET: j <= P - i, k <= P - i - j; PYR: j <= P, k <= P - max(i, j); modes are numbered i slowest, k
-// fastest (the host's mode order).
 template <int P, int TYPE>
 struct Collapsed {
     static constexpr int n = P + 1;
@@ -1087,20 +1194,13 @@ __global__ void reset_counters(unsigned int* c)
         c[threadIdx.x] = 0u;
 }
 
+// Face tiles of one size class cost the same, so the persistent face grid takes them round-robin (no work
+// counter: one atomic per small tile serializes on its address).
 template <int FLUX>
 __global__ __launch_bounds__(kBlock) void face_persistent(MixArgs a)
 {
-    __shared__ int s_t;
-    for (;;) {
-        if (threadIdx.x == 0)
-            s_t = (int)atomicAdd(a.counter, 1u);
-        __syncthreads();
-        const int t = s_t;
-        __syncthreads();
-        if (t >= a.n_ftiles)
-            break;
+    for (int64_t t = blockIdx.x; t < a.n_ftiles; t += gridDim.x)
         face_tile<FLUX>(a, a.ftiles[t]);
-    }
 }
 
 template <int FLUX>
@@ -1109,20 +1209,48 @@ __global__ __launch_bounds__(kBlock) void face_range(MixArgs a, int64_t t0)
     face_tile<FLUX>(a, a.ftiles[t0 + blockIdx.x]);
 }
 
+// Volume tiles are taken kVolChunk at a time from the work counter (largest first), so the counter sees
+// one atomic per kVolChunk tiles.
+constexpr int kVolChunk = 4;
+
+__device__ __attribute__((always_inline)) inline void vol_persistent_body(const MixArgs& a, double* lds, int* s_t)
+{
+    for (;;) {
+        if (threadIdx.x == 0)
+            *s_t = (int)atomicAdd(a.counter + 1, (unsigned)kVolChunk);
+        __syncthreads();
+        const int t0 = *s_t;
+        __syncthreads();
+        if (t0 >= a.n_vtiles)
+            break;
+        const int t1 = t0 + kVolChunk < a.n_vtiles ? t0 + kVolChunk : (int)a.n_vtiles;
+        for (int t = t0; t < t1; ++t)
+            vol_dispatch(a, a.vtiles[a.vorder[t]], lds);
+    }
+}
+
+// The persistent volume kernel holds every (type, p) path, so its register count is the largest path's
+// plus what the tile loop keeps live; three variants trade that for occupancy (2, 3 or 4 waves per SIMD,
+// the last two with a few spilled registers) and the autotuner picks one.
 __global__ __launch_bounds__(kBlock) void vol_persistent(MixArgs a)
 {
     extern __shared__ double lds[];
     __shared__ int s_t;
-    for (;;) {
-        if (threadIdx.x == 0)
-            s_t = (int)atomicAdd(a.counter + 1, 1u);
-        __syncthreads();
-        const int t = s_t;
-        __syncthreads();
-        if (t >= a.n_vtiles)
-            break;
-        vol_dispatch(a, a.vtiles[a.vorder[t]], lds);
-    }
+    vol_persistent_body(a, lds, &s_t);
+}
+
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void vol_persistent_w3(MixArgs a)
+{
+    extern __shared__ double lds[];
+    __shared__ int s_t;
+    vol_persistent_body(a, lds, &s_t);
+}
+
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void vol_persistent_w4(MixArgs a)
+{
+    extern __shared__ double lds[];
+    __shared__ int s_t;
+    vol_persistent_body(a, lds, &s_t);
 }
 
 __global__ __launch_bounds__(kBlock) void vol_range(MixArgs a, int64_t t0)
@@ -1170,6 +1298,7 @@ struct dgx_mixed_ctx {
     int n_group = 0;
     int mode = DGX_MIXED_LAUNCH_PERSISTENT;
     int bpc = 2;
+    int pwaves = 2;                      // occupancy variant of the persistent volume kernel (waves/SIMD)
     std::vector<int32_t> tiles_of[2];    // tile shapes of each launch form
     int n_cu = 0;
     std::vector<GroupDev> groups;
@@ -1180,6 +1309,7 @@ struct dgx_mixed_ctx {
            *fw = nullptr;
     int64_t *dof = nullptr, *fb_off = nullptr, *fop_off = nullptr, *fopt_off = nullptr, *fw_off = nullptr,
             *fopr_off = nullptr, *foprt_off = nullptr;
+    int32_t *fopr_n = nullptr;
     int32_t *ef = nullptr, *elf = nullptr, *face_elem = nullptr, *face_op = nullptr, *face_wt = nullptr, *fop_nq = nullptr,
             *fop_ns = nullptr, *face_order = nullptr, *vorder = nullptr;
     GroupDev* d_groups = nullptr;
@@ -1232,7 +1362,9 @@ int64_t tp_len(const GroupDev& g)
         const int64_t sc = g.type == DGX_ELEM_TET ? n * n * n * n : n * n * n;
         return 2 * n * n + 2 * sb + 2 * sc + 3 * n + n * n * n + g.ns;
     }
-    return g.type == DGX_ELEM_WEDGE ? 2 * n * n + n + 3 * n * n * nt + n * n : 2 * n * n + 3 * n;
+    // WEDGE: B1, D1, w1, Bt, Gr, Gs, wt, then the edge matrices E [3][n][Nt] and tm, tp [n]
+    return g.type == DGX_ELEM_WEDGE ? 2 * n * n + n + 3 * n * n * nt + n * n + 3 * n * nt + 2 * n
+                                    : 2 * n * n + 3 * n;
 }
 
 bool tile_ok(const GroupDev& g, int tile)
@@ -1322,6 +1454,7 @@ MixArgs make_args(dgx_mixed_ctx* c, const double* u, const double* reg, double* 
     a.fopt_off = c->fopt_off;
     a.fopr_off = c->fopr_off;
     a.foprt_off = c->foprt_off;
+    a.fopr_n = c->fopr_n;
     a.face_lf_of = c->elf;
     a.fop = c->fop;
     a.fw_off = c->fw_off;
@@ -1365,7 +1498,12 @@ hipError_t launch_vol(dgx_mixed_ctx* c, const MixArgs& a, int mode, int only_gro
 {
     if (mode == DGX_MIXED_LAUNCH_PERSISTENT && only_group < 0) {
         const int grid = (int)std::min<int64_t>((int64_t)c->n_cu * c->bpc, std::max<int64_t>(c->n_vtiles, 1));
-        vol_persistent<<<grid, kBlock, c->lds_max, c->stream>>>(a);
+        if (c->pwaves == 4)
+            vol_persistent_w4<<<grid, kBlock, c->lds_max, c->stream>>>(a);
+        else if (c->pwaves == 3)
+            vol_persistent_w3<<<grid, kBlock, c->lds_max, c->stream>>>(a);
+        else
+            vol_persistent<<<grid, kBlock, c->lds_max, c->stream>>>(a);
         return hipGetLastError();
     }
     for (int g = 0; g < c->n_group; ++g) {
@@ -1512,7 +1650,7 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     c->n_group = m->n_group;
     MIX_CHECK(c, m->group_start[0] == 0 && m->group_start[m->n_group] == m->n_elem, "group_start range");
     // groups and dof offsets
-    std::vector<int32_t> ns_of(c->E), type_of(c->E);
+    std::vector<int32_t> ns_of(c->E), type_of(c->E), group_of(c->E);
     std::vector<int64_t> dof(c->E + 1, 0);
     int64_t vol_len = 0;
     c->groups.resize(c->n_group);
@@ -1545,6 +1683,7 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
         for (int64_t e = G.start; e < G.start + G.count; ++e) {
             ns_of[e] = G.ns;
             type_of[e] = G.type;
+            group_of[e] = g;
         }
     }
     for (int64_t e = 0; e < c->E; ++e)
@@ -1564,15 +1703,13 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     std::vector<int64_t> froff(m->n_face_op, -1), frofft(m->n_face_op, -1);
     std::vector<int32_t> fn2(m->n_face_op, 0);
     if (m->fop_red_off) {
+        MIX_CHECK(c, m->fop_red_n, "face-modal rows without their widths");
         for (int o = 0; o < m->n_face_op; ++o) {
             froff[o] = m->fop_red_off[o];
             if (froff[o] < 0)
                 continue;
-            int n = 1;
-            while (n * n * n < fns[o])
-                ++n;
-            MIX_CHECK(c, n * n * n == fns[o], "face-modal rows given for a side that is not HEX");
-            fn2[o] = n * n;
+            fn2[o] = m->fop_red_n[o];
+            MIX_CHECK(c, fn2[o] > 0 && fn2[o] <= 64 && fn2[o] < fns[o], "face-modal row width");
             fop_len = std::max<int64_t>(fop_len, froff[o] + (int64_t)fnq[o] * fn2[o]);
         }
     }
@@ -1597,11 +1734,12 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     }
     // faces: sides, operators, element face lists, flux buffer offsets
     std::vector<int32_t> ef(6 * c->E, -1), elf(6 * c->E, -1), nfe(c->E, 0);
+    // elements whose lifts and traces run in face-modal form: sum-factorized HEX and WEDGE groups
     std::vector<char> hex_tp(c->E, 0);
     for (int g = 0; g < c->n_group; ++g)
-        if (c->groups[g].tp_off >= 0 && c->groups[g].type == DGX_ELEM_HEX)
+        if (c->groups[g].tp_off >= 0 && (c->groups[g].type == DGX_ELEM_HEX || c->groups[g].type == DGX_ELEM_WEDGE))
             for (int64_t e = c->groups[g].start; e < c->groups[g].start + c->groups[g].count; ++e)
-                hex_tp[e] = 1;
+                hex_tp[e] = (char)(1 + c->groups[g].type);
     std::vector<int64_t> fb_off(c->F);
     int64_t fb = 0;
     c->face_lg_of.resize(c->F);
@@ -1619,11 +1757,16 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
             const int32_t lf = m->face_lf ? m->face_lf[2 * f + s] : -1;
             MIX_CHECK(c, nfe[e] < kFacesOf[type_of[e]], "element with more faces than its type has");
             if (hex_tp[e]) {
-                // the sum-factorized HEX path lifts and traces in face-modal form
-                MIX_CHECK(c, lf >= 0 && lf < 6 && froff[m->face_op[2 * f + s]] >= 0,
-                          "HEX sides of a sum-factorized group need face_lf and face-modal rows");
+                // the sum-factorized HEX and WEDGE paths lift and trace in face-modal form
+                const int nlf = hex_tp[e] == 1 + DGX_ELEM_HEX ? 6 : 5;
+                const int o = m->face_op[2 * f + s];
+                MIX_CHECK(c, lf >= 0 && lf < nlf && froff[o] >= 0,
+                          "HEX / WEDGE sides of a sum-factorized group need face_lf and face-modal rows");
+                const int n = c->groups[group_of[e]].p + 1;
+                MIX_CHECK(c, fn2[o] == ((hex_tp[e] == 1 + DGX_ELEM_WEDGE && lf >= 3) ? n * (n + 1) / 2 : n * n),
+                          "face-modal row width differs from the side's face");
                 for (int k = 0; k < nfe[e]; ++k)
-                    MIX_CHECK(c, elf[6 * e + k] != lf, "two faces of one HEX on the same local face");
+                    MIX_CHECK(c, elf[6 * e + k] != lf, "two faces of one element on the same local face");
             }
             elf[6 * e + nfe[e]] = lf;
             ef[6 * e + nfe[e]++] = (int32_t)(2 * f + s);
@@ -1687,6 +1830,7 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     MIX_RUN(upload(c, &c->fopt_off, fofft));
     MIX_RUN(upload(c, &c->fopr_off, froff));
     MIX_RUN(upload(c, &c->foprt_off, frofft));
+    MIX_RUN(upload(c, &c->fopr_n, fn2));
     MIX_RUN(upload(c, &c->elf, elf));
     MIX_RUN(upload(c, &c->fop, fop));
     MIX_RUN(upload(c, &c->fw_off, std::vector<int64_t>(m->fw_off, m->fw_off + m->n_face_wt)));
@@ -1799,6 +1943,11 @@ const char* dgx_mixed_last_error(const dgx_mixed_ctx* ctx)
 int64_t dgx_mixed_n_dof(const dgx_mixed_ctx* ctx)
 {
     return ctx ? ctx->n_dof : -1;
+}
+
+int dgx_mixed_persistent_waves(const dgx_mixed_ctx* ctx)
+{
+    return ctx ? ctx->pwaves : -1;
 }
 
 dgx_status dgx_mixed_set_state(dgx_mixed_ctx* ctx, int ptr_kind, const double* u)
@@ -1917,7 +2066,7 @@ dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings)
     //    (the per-group best tile, halved until it fits) and the budget and grid size are searched
     const size_t caps[] = { 24 << 10, 32 << 10, 40 << 10, 48 << 10, (size_t)kMaxTileLds };
     std::vector<int32_t> best_tiles = pg;
-    int best_bpc = ctx->bpc;
+    int best_bpc = ctx->bpc, best_w = ctx->pwaves;
     double best_us = 1e300;
     for (size_t cap : caps) {
         std::vector<int32_t> tl(ctx->n_group);
@@ -1936,22 +2085,27 @@ dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings)
         MIX_RUN(build_vtiles(ctx));
         a = make_args(ctx, ctx->u, nullptr, ctx->rhs, -1, 1, 0.0);
         const int max_bpc = std::max(1, std::min(8, (int)((160u << 10) / ctx->lds_max)));
-        for (int b = 1; b <= max_bpc; ++b) {
-            ctx->bpc = b;
-            double us = 0.0;
-            MIX_RUN(time_launches(ctx, reps, [&]() {
-                reset_counters<<<1, 64, 0, ctx->stream>>>(ctx->counter);
-                return launch_vol(ctx, a, DGX_MIXED_LAUNCH_PERSISTENT);
-            }, &us));
-            if (us < best_us) {
-                best_us = us;
-                best_bpc = b;
-                best_tiles = tl;
+        for (int w = 2; w <= 4; ++w) {
+            ctx->pwaves = w;
+            for (int b = 1; b <= max_bpc; ++b) {
+                ctx->bpc = b;
+                double us = 0.0;
+                MIX_RUN(time_launches(ctx, reps, [&]() {
+                    reset_counters<<<1, 64, 0, ctx->stream>>>(ctx->counter);
+                    return launch_vol(ctx, a, DGX_MIXED_LAUNCH_PERSISTENT);
+                }, &us));
+                if (us < best_us) {
+                    best_us = us;
+                    best_bpc = b;
+                    best_w = w;
+                    best_tiles = tl;
+                }
             }
         }
     }
     ctx->tiles_of[DGX_MIXED_LAUNCH_PERSISTENT] = best_tiles;
     ctx->bpc = best_bpc;
+    ctx->pwaves = best_w;
     for (int g = 0; g < ctx->n_group; ++g)
         ctx->groups[g].tile = ctx->tiles_of[ctx->mode][g];
     MIX_RUN(build_vtiles(ctx));

@@ -472,3 +472,59 @@ def collapse_chain(etype, a, b, c):
         da = (2.0 / (1 - c), 0.0, a / (1 - c))
         db = (0.0, 2.0 / (1 - c), b / (1 - c))
     return da, db
+
+
+def tri_basis(p, x):
+    """The orthonormal TRI basis of tri_operators at points x (N, 2)."""
+    xq, w = face_rule(F_TRI, p + 1)
+    V = _tri_raw(p, xq)
+    R = np.linalg.inv(np.linalg.cholesky(V.T @ (w[:, None] * V)).T)
+    return _tri_raw(p, np.asarray(x, dtype=float)) @ R
+
+
+def _wedge_edge(lf):
+    """Endpoints A, B (r, s) of the TRI edge under WEDGE quad face lf, in the face's vertex order."""
+    fv = FACES[WEDGE][lf]
+    return VERTS[WEDGE][fv[0]][:2], VERTS[WEDGE][fv[1]][:2]
+
+
+@lru_cache(maxsize=None)
+def wedge_edge_matrix(p, lf):
+    """E [n][Nt]: the TRI modes restricted to the edge of WEDGE quad face lf, in the orthonormal
+    Legendre basis of the edge parameter xi in [-1, 1] (A -> B): psi_m(edge(xi)) = sum_a E[a][m] L_a(xi)."""
+    n = p + 1
+    A, B = _wedge_edge(lf)
+    xg, wg = gauss_jacobi(n, 0.0, 0.0)
+    pts = A[None, :] + np.outer((xg + 1) / 2, B - A)
+    psi = tri_basis(p, pts)                                   # [g][m]
+    L = np.stack([math.sqrt((2 * a + 1) / 2.0) * _leg(a, xg) for a in range(n)], axis=1)   # [g][a]
+    return L.T @ (wg[:, None] * psi)
+
+
+def wedge_face_modal(p, lf, case, q, Bf):
+    """The face-modal rows of a WEDGE face operator Bf [Nq][Ns] (modes k Nt + m, TRI fastest):
+    quad face (lf 0..2): BF [Nq][n^2], BF[q][a + n k] = L_a(xi_q) L_k(t_q), with
+    Bf[q][(k, m)] = sum_a BF[q][a + n k] E[a][m] (E = wedge_edge_matrix);  TRI face (lf 3, 4: t = -1, +1):
+    BF [Nq][Nt] = psi_m at the face nodes, with Bf[q][(k, m)] = tau_k BF[q][m], tau = L_k(-1) or L_k(+1).
+    Checks the factorization."""
+    n = p + 1
+    nt = n * (n + 1) // 2
+    ft = face_type(len(FACES[WEDGE][lf]))
+    pts2, _ = face_rule(ft, q + 1)
+    x = face_map(WEDGE, lf, perms_of(ft)[case], pts2)          # reference coordinates of the face nodes
+    Lt = np.stack([math.sqrt((2 * k + 1) / 2.0) * _leg(k, x[:, 2]) for k in range(n)], axis=1)   # [q][k]
+    B3 = Bf.reshape(-1, n, nt)                                 # [q][k][m]
+    if lf < 3:
+        A, Bv = _wedge_edge(lf)
+        d = Bv - A
+        xi = 2.0 * ((x[:, :2] - A) @ d) / (d @ d) - 1.0
+        La = np.stack([math.sqrt((2 * a + 1) / 2.0) * _leg(a, xi) for a in range(n)], axis=1)   # [q][a]
+        BF = (La[:, :, None] * Lt[:, None, :]).transpose(0, 2, 1).reshape(-1, n * n)        # [q][k][a] -> a + n k
+        E = wedge_edge_matrix(p, lf)
+        rebuilt = np.einsum("qka,am->qkm", BF.reshape(-1, n, n), E)
+    else:
+        BF = tri_basis(p, x[:, :2])
+        tau = line_traces(p)[lf - 3]
+        rebuilt = tau[None, :, None] * BF[:, None, :]
+    assert np.abs(rebuilt - B3).max() < 1e-11, "WEDGE face operator does not factor"
+    return BF

@@ -69,7 +69,9 @@ def build_tables(mesh, sum_factorized=True):
                 tp += [tm, tq]
             if t == M.WEDGE:
                 Bt, Gr, Gs, wt = M.tri_operators(p)
-                tp += [Bt.ravel(), Gr.ravel(), Gs.ravel(), wt]
+                tm, tq = M.line_traces(p)
+                tp += [Bt.ravel(), Gr.ravel(), Gs.ravel(), wt,
+                       np.concatenate([M.wedge_edge_matrix(p, e).ravel() for e in range(3)]), tm, tq]
             tp = np.concatenate(tp)
             vol_tp_off.append(pos)
             vol.append(tp)
@@ -77,7 +79,7 @@ def build_tables(mesh, sum_factorized=True):
         else:
             vol_tp_off.append(-1)
     # faces
-    fop_index, fop_nq, fop_ns, fop_off, fop_red_off, fop_data = {}, [], [], [], [], []
+    fop_index, fop_nq, fop_ns, fop_off, fop_red_off, fop_red_n, fop_data = {}, [], [], [], [], [], []
     fw_index, fw_n, fw_off, fw_data = {}, [], [], []
     fpos = wpos = 0
     F = mesh.n_face
@@ -98,13 +100,15 @@ def build_tables(mesh, sum_factorized=True):
                 fop_off.append(fpos)
                 fop_data.append(Bf.ravel())
                 fpos += Bf.size
-                if t == M.HEX and sum_factorized:
-                    BF = M.hex_face_modal(p, lf, Bf)
+                if t in (M.HEX, M.WEDGE) and sum_factorized:
+                    BF = (M.hex_face_modal(p, lf, Bf) if t == M.HEX else M.wedge_face_modal(p, lf, case, q, Bf))
                     fop_red_off.append(fpos)
+                    fop_red_n.append(BF.shape[1])
                     fop_data.append(BF.ravel())
                     fpos += BF.size
                 else:
                     fop_red_off.append(-1)
+                    fop_red_n.append(0)
             face_op[f, s] = fop_index[k]
         ft = int(mesh.face_type[f])
         kw = (ft, q)
@@ -129,6 +133,7 @@ def build_tables(mesh, sum_factorized=True):
         "face_jac": np.ascontiguousarray(mesh.face_jac, dtype=np.float64),
         "fop_nq": np.array(fop_nq, dtype=np.int32), "fop_ns": np.array(fop_ns, dtype=np.int32),
         "fop_off": np.array(fop_off, dtype=np.int64), "fop_red_off": np.array(fop_red_off, dtype=np.int64),
+        "fop_red_n": np.array(fop_red_n, dtype=np.int32),
         "fop_data": np.concatenate(fop_data),
         "face_lf": np.ascontiguousarray(mesh.face_lf, dtype=np.int32),
         "fw_n": np.array(fw_n, dtype=np.int32), "fw_off": np.array(fw_off, dtype=np.int64),
@@ -240,6 +245,10 @@ class MixedSolver:
         t = np.empty(self.tab.n_group)
         self._check(self._lib.dgx_mixed_autotune(self._ctx, int(reps), _ffi.dptr(t)))
         return t
+
+    def persistent_waves(self):
+        """Occupancy variant (waves per SIMD) of the persistent volume kernel the autotuner chose."""
+        return int(self._lib.dgx_mixed_persistent_waves(self._ctx))
 
     def time_stage(self, dt, nstages=30):
         """(stage, face kernel, volume kernel) mean microseconds."""

@@ -193,8 +193,9 @@ dgx_status dgx_jacobian_blocks_f32(dgx_ctx* ctx, int apply_minv, float* blocks_t
 
 /* Batched block inversion for a block-Jacobi preconditioner: inv[b] = (shift[b] I - blocks[b])^-1 for
  * n_blocks dense n x n blocks (n <= 320), blocks row-major FP64, inv row-major FP32, all device
- * pointers; singular[b] = 0, or k + 1 when column k had no non-zero pivot (the block's output is then
- * meaningless).  Gauss-Jordan with partial pivoting, one workgroup per block, launched on `stream`
+ * pointers; singular[b] = 0, k + 1 when column k had no non-zero pivot, or n + 1 when the block or its
+ * inverse holds a non-finite value (a NaN input, or a denormal pivot whose reciprocal overflows): the
+ * block's output is then meaningless.  Gauss-Jordan with partial pivoting, one workgroup per block, launched on `stream`
  * (a hipStream_t; NULL = the null stream) without synchronising.  Replaces the preconditioner set-up of
  * the reference's KSP (constructor_petsc_ksp, solve_implicit.c:562-590: ILU on the assembled matrix),
  * which a matrix-free solver cannot factor; the blocks come from dgx_jacobian_blocks. */

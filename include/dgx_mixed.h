@@ -50,14 +50,18 @@ typedef struct dgx_mixed_mesh {
      * the reference's cv0_vs_vc and tw1_vt_vc (element_operators.c) */
     const double* vol_ops;
     const int64_t* vol_off;      /* [n_group] */
-    /* nullable; HEX and WEDGE groups: offset in vol_ops of the tensor-product factors, -1 for the dense
-     * path.  HEX: B1 [n][n] (orthonormal Legendre at the n = p+1 Gauss-Legendre nodes), D1 [n][n] (their
-     * derivatives), w1 [n] (weights), with B = B1 (x) B1 (x) B1 (r fastest) and G_d the same with D1 in
-     * direction d, then tm, tp [n] (L_i(-1), L_i(+1)).  WEDGE: B1, D1, w1, then Bt, Gr, Gs [n^2][Nt]
-     * (orthonormal TRI basis at the collapsed
-     * TRI nodes and its r, s derivatives, Nt = n(n+1)/2) and wt [n^2], with B = B1 (x) Bt (TRI fastest).
-     * The kernels then apply the volume operators by sum factorization (element_operators_tp.c:90-241
-     * builds the reference's tensor-product operators from the same factors). */
+    /* nullable; offset in vol_ops of a group's factor tables, -1 for the dense path.  The kernels then
+     * apply the volume operators by sum factorization (element_operators_tp.c:90-241 builds the
+     * reference's tensor-product operators from the same LINE / TRI factors).
+     * HEX: B1 [n][n] (orthonormal Legendre at the n = p+1 Gauss-Legendre nodes), D1 [n][n] (their
+     * derivatives), w1 [n] (weights), tm, tp [n] (L_i(-1), L_i(+1)); B = B1 (x) B1 (x) B1 (r fastest), G_d
+     * the same with D1 in direction d.
+     * WEDGE: B1, D1, w1, then Bt, Gr, Gs [n^2][Nt] (orthonormal TRI basis at the collapsed TRI nodes and its
+     * r, s derivatives, Nt = n(n+1)/2), wt [n^2], E [3][n][Nt] (the TRI modes on the edge under quad face e
+     * in the edge's Legendre basis), tm, tp [n]; B = B1 (x) Bt (TRI fastest).
+     * TET, PYR (collapsed coordinates a, b, c): A, Ad [n][n]; B, Bd (TET [n][n][n] as [i][q][j], PYR
+     * [n][n]); C, Cd (TET [n][n][n][n] as [i][j][r][k], PYR [n][n][n] as [max(i,j)][r][k]); the nodes
+     * xa, xb, xc [n]; the node weights [n^3]; the mode scales s [Ns]: phi = s A B C. */
     const int64_t* vol_tp_off;
     const double* jdet;          /* [n_elem] det of the affine map */
     const double* metrics;       /* [n_elem][3][3] m(i,j) = J d xi_j / d x_i (metrics_vc, constant) */
@@ -65,7 +69,7 @@ typedef struct dgx_mixed_mesh {
      * face Jacobian = physical face area / reference face area */
     const int32_t* face_elem;    /* [n_face][2] */
     const int32_t* face_lf;      /* [n_face][2] local face of each side (neigh_info[s].ind_lf); nullable
-                                  * unless HEX groups are sum-factorized */
+                                  * unless HEX or WEDGE groups are sum-factorized */
     const int32_t* face_op;      /* [n_face][2] face operator of each side */
     const int32_t* face_wt;      /* [n_face]    face weight table */
     const double* face_normal;   /* [n_face][3] */
@@ -77,11 +81,14 @@ typedef struct dgx_mixed_mesh {
     const int32_t* fop_nq;       /* [n_face_op] */
     const int32_t* fop_ns;       /* [n_face_op] */
     const int64_t* fop_off;      /* [n_face_op] */
-    /* nullable; HEX sides: offset in fop_data of the face-modal rows BF [Nq][n^2] with
-     * B_f[q][(i, a, b)] = tau_i BF[q][a + n b], tau = L_i(-1) or L_i(+1) along the face's normal
-     * direction and (a, b) the other two directions, lower one fastest (required for the HEX sides of
-     * sum-factorized groups: their lifts and traces contract the face-modal form); -1 otherwise */
+    /* nullable; offset in fop_data of a side's face-modal rows BF, -1 if none (required for the sides of
+     * sum-factorized HEX and WEDGE groups: their lifts and traces contract the face-modal form).
+     * HEX: BF [Nq][n^2] with B_f[q][(i, a, b)] = tau_i BF[q][a + n b], tau = L_i(-1) or L_i(+1) along the
+     * face's normal direction and (a, b) the other two directions, lower one fastest.  WEDGE quad face
+     * lf (0..2, over TRI edge lf): BF [Nq][n^2] with B_f[q][(k, m)] = sum_a BF[q][a + n k] E_lf[a][m];
+     * WEDGE TRI face (lf 3, 4: t = -1, +1): BF [Nq][Nt] with B_f[q][(k, m)] = tau_k BF[q][m]. */
     const int64_t* fop_red_off;
+    const int32_t* fop_red_n;    /* [n_face_op] columns of the face-modal rows (nullable with fop_red_off) */
     const double* fop_data;
     const int32_t* fw_n;         /* [n_face_wt] face cubature nodes */
     const int64_t* fw_off;       /* [n_face_wt] */
@@ -122,10 +129,13 @@ dgx_status dgx_mixed_get_launch(const dgx_mixed_ctx* ctx, int* mode, int32_t* ti
  * of that group alone at each legal tile_elems (reps launches each, HIP events on the context's stream)
  * and keeps the fastest for the per-group form; for the persistent form, whose one launch has one LDS
  * size, caps those tiles to an LDS budget (24..60 KB, halving a tile until it fits) and times the whole
- * persistent volume launch for every budget and every blocks_per_cu the budget lets fit on a CU,
- * keeping the fastest.  Runs the residual form of the kernels: the state is untouched.  timings
+ * persistent volume launch for every budget, every blocks_per_cu the budget lets fit on a CU and
+ * every occupancy variant of the kernel (dgx_mixed_persistent_waves), keeping the fastest.  Runs the residual form of the kernels: the state is untouched.  timings
  * (nullable) receives per group the best microseconds per launch. */
 dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings);
+/* The occupancy variant of the persistent volume kernel the autotuner chose: 2, 3 or 4 waves per SIMD
+ * (the kernel holds every (type, p) path; the 3- and 4-wave variants cap its registers). */
+int dgx_mixed_persistent_waves(const dgx_mixed_ctx* ctx);
 
 /* Mean microseconds per stage of nstages SSP-RK3 stages (graph replays, events on the context's
  * stream), and of its kernels: [0] stage, [1] face kernel, [2] volume kernel (each timed alone). */

@@ -103,6 +103,36 @@ def test_singular_block_reported_and_neighbours_intact():
         implicit.block_inverse(Bt, st)
 
 
+def test_nonfinite_and_tiny_pivot_blocks_flagged():
+    """A block with one NaN entry and a block whose only pivot is an FP32 denormal (its reciprocal
+    overflows) are flagged n + 1; the other blocks of the batch are unaffected."""
+    _require_gpu()
+    import torch
+    print("case: non-finite / tiny-pivot blocks", flush=True)
+    rng = np.random.default_rng(9)
+    n, nb = 24, 4
+    B = rng.standard_normal((nb, n, n)) / np.sqrt(n)
+    shift = np.full(nb, 2.0)
+    B[1, 3, 7] = np.nan
+    B[2] = 0.0
+    shift[2] = 1e-40                       # shift I - B = 1e-40 I: FP32 denormal pivots
+    Bt = torch.from_numpy(B).cuda()
+    st = torch.from_numpy(shift).cuda()
+    out = torch.empty((nb, n, n), dtype=torch.float32, device="cuda")
+    sing = torch.full((nb,), -1, dtype=torch.int32, device="cuda")
+    lib = _ffi.load()
+    status = lib.dgx_block_inverse(nb, n, ctypes.c_void_p(Bt.data_ptr()), ctypes.c_void_p(st.data_ptr()),
+                                   ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(sing.data_ptr()), None)
+    assert status == _ffi.DGX_OK
+    torch.cuda.synchronize()
+    s = sing.cpu().numpy()
+    assert s[1] != 0 and s[2] != 0 and s[0] == 0 and s[3] == 0, s
+    X = out.cpu().numpy().astype(np.float64)
+    for b in (0, 3):
+        ref = np.linalg.inv(shift[b] * np.eye(n) - B[b])
+        assert np.linalg.norm(X[b] - ref) / np.linalg.norm(ref) <= 1e-5
+
+
 def test_argument_checks():
     _require_gpu()
     import torch
