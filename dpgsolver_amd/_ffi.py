@@ -78,7 +78,7 @@ class DgxMixedMesh(ctypes.Structure):
                 ("fw_data", c_double_p)]
 
 
-MIXED_LAUNCH_PERSISTENT, MIXED_LAUNCH_PER_GROUP = 0, 1
+MIXED_LAUNCH_PERSISTENT, MIXED_LAUNCH_PER_GROUP, MIXED_LAUNCH_GROUPED = 0, 1, 2
 
 
 class DgxError(RuntimeError):

@@ -860,12 +860,16 @@ static dgx_status lin_apply(dgx_ctx* c, bool adjoint, int apply_minv, int layout
         HIP_TRY(c, hipMalloc(&c->lin_tr, tbytes));
     }
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    dgx_status st = upload_state(c, layout, ptr_kind, in, c->lin_x);
-    if (st != DGX_OK)
-        return st;
-    if (adjoint && apply_minv)
-        HIP_TRY(c, dgx::launch_scale_by_imass(c->lin_x, c->imass, c->lin_x, c->Ep, c->nv, c->ns, c->stream));
-    HIP_TRY(c, dgx::launch_trace(c->d, c->p, c->lin_x, c->lin_tr, c->Ep, c->stream));
+    const double* src = in;
+    if (ptr_kind == DGX_PTR_HOST) {
+        HIP_TRY(c, hipMemcpyAsync(c->stage_buf, in, sizeof(double) * (size_t)c->E * c->nv * c->ns,
+                                  hipMemcpyHostToDevice, c->stream));
+        src = c->stage_buf;
+    }
+    // the input in slot order (times M^-1 for the adjoint of M^-1 R) and its face traces, one pass
+    HIP_TRY(c, dgx::launch_upload_trace(c->d, c->p, src, layout, c->slot_elem, (adjoint && apply_minv) ? c->imass : nullptr,
+                                        c->lin_x, c->lin_tr, c->E, c->Ep, c->stream));
+    dgx_status st = DGX_OK;
     dgx::LinArgs a;
     std::memset(&a, 0, sizeof(a));
     a.u = c->state_buf ? c->up : c->u;

@@ -13,6 +13,9 @@ int64_t stage_grid(int d, int p, int64_t n_elem);
 hipError_t upload_ops(int p, const Ops1D& ops);
 hipError_t launch_stage(int d, int p, int flux, const StageArgs& args, hipStream_t st);
 hipError_t launch_trace(int d, int p, const double* u, double* tr, int64_t E, hipStream_t st);
+// caller-layout src -> slot-order dst (times imass if given) and its face traces, in one pass
+hipError_t launch_upload_trace(int d, int p, const double* src, int layout, const int32_t* slot_elem,
+                               const double* imass, double* dst, double* tr, int64_t E, int64_t Ep, hipStream_t st);
 hipError_t launch_to_internal(const double* src, int layout, double* dst, const int32_t* slot_elem, int64_t E,
                               int64_t Ep, int nv, int ns, hipStream_t st);
 hipError_t launch_from_internal(const double* src, double* dst, int layout, const int32_t* elem_slot, int64_t E,

@@ -16,7 +16,8 @@
 // The state is read once per stage (by the volume kernel); the face kernel reads traces only.  A trace
 // pass (the volume kernel's last phase alone) fills the trace buffer after a state upload.
 // Launch forms: a persistent grid whose workgroups take tiles from a work counter (tiles of every
-// group in one launch, largest first), or one launch per (type, p) group / per face size class.  The
+// group in one launch, largest first), one grouped launch with a workgroup per tile of every group, or
+// one launch per (type, p) group / per face size class.  The
 // volume kernel updates its elements in place (an element reads only its own coefficients; neighbour
 // data come through the trace and face flux buffers).
 #include <hip/hip_runtime.h>
@@ -1253,6 +1254,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     vol_persistent_body(a, lds, &s_t);
 }
 
+// The grouped form: one launch over the tiles of every group, one tile per workgroup, in the
+// persistent order (largest first); the hardware dispatcher balances the load, and without a work loop
+// the kernel keeps the register count of a single tile.
+__global__ __launch_bounds__(kBlock) void vol_grouped(MixArgs a)
+{
+    extern __shared__ double lds[];
+    vol_dispatch(a, a.vtiles[a.vorder[blockIdx.x]], lds);
+}
+
 __global__ __launch_bounds__(kBlock) void vol_range(MixArgs a, int64_t t0)
 {
     extern __shared__ double lds[];
@@ -1299,7 +1309,7 @@ struct dgx_mixed_ctx {
     int mode = DGX_MIXED_LAUNCH_PERSISTENT;
     int bpc = 2;
     int pwaves = 2;                      // occupancy variant of the persistent volume kernel (waves/SIMD)
-    std::vector<int32_t> tiles_of[2];    // tile shapes of each launch form
+    std::vector<int32_t> tiles_of[3];    // tile shapes of each launch form
     int n_cu = 0;
     std::vector<GroupDev> groups;
     std::vector<int32_t> face_lg_of;     // per face
@@ -1471,6 +1481,13 @@ MixArgs make_args(dgx_mixed_ctx* c, const double* u, const double* reg, double* 
 
 hipError_t launch_face(dgx_mixed_ctx* c, const MixArgs& a, int mode)
 {
+    if (mode == DGX_MIXED_LAUNCH_GROUPED && c->n_ftiles > 0) {
+        if (c->flux == 0)
+            face_range<0><<<(unsigned)c->n_ftiles, kBlock, 0, c->stream>>>(a, 0);
+        else
+            face_range<1><<<(unsigned)c->n_ftiles, kBlock, 0, c->stream>>>(a, 0);
+        return hipGetLastError();
+    }
     if (mode == DGX_MIXED_LAUNCH_PERSISTENT) {
         const int grid = (int)std::min<int64_t>((int64_t)c->n_cu * c->bpc, std::max<int64_t>(c->n_ftiles, 1));
         if (c->flux == 0)
@@ -1504,6 +1521,11 @@ hipError_t launch_vol(dgx_mixed_ctx* c, const MixArgs& a, int mode, int only_gro
             vol_persistent_w3<<<grid, kBlock, c->lds_max, c->stream>>>(a);
         else
             vol_persistent<<<grid, kBlock, c->lds_max, c->stream>>>(a);
+        return hipGetLastError();
+    }
+    if (mode == DGX_MIXED_LAUNCH_GROUPED && only_group < 0) {
+        if (c->n_vtiles > 0)
+            vol_grouped<<<(unsigned)c->n_vtiles, kBlock, c->lds_max, c->stream>>>(a);
         return hipGetLastError();
     }
     for (int g = 0; g < c->n_group; ++g) {
@@ -1847,7 +1869,7 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     MIX_TRY(c, hipMemset(c->u, 0, c->n_dof * sizeof(double)));
     MIX_TRY(c, hipMemset(c->up, 0, c->n_dof * sizeof(double)));
     MIX_TRY(c, hipMemset(c->tr, 0, 2 * c->fb_len * sizeof(double)));
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 3; ++k) {
         c->tiles_of[k].resize(c->n_group);
         for (int g = 0; g < c->n_group; ++g)
             c->tiles_of[k][g] = c->groups[g].tile;
@@ -1994,7 +2016,8 @@ dgx_status dgx_mixed_set_launch(dgx_mixed_ctx* ctx, int mode, const int32_t* til
 {
     if (!ctx)
         return DGX_ERR_INVALID_ARG;
-    MIX_CHECK(ctx, mode == DGX_MIXED_LAUNCH_PERSISTENT || mode == DGX_MIXED_LAUNCH_PER_GROUP, "launch mode");
+    MIX_CHECK(ctx, mode == DGX_MIXED_LAUNCH_PERSISTENT || mode == DGX_MIXED_LAUNCH_PER_GROUP ||
+                       mode == DGX_MIXED_LAUNCH_GROUPED, "launch mode");
     MIX_CHECK(ctx, blocks_per_cu >= 1 && blocks_per_cu <= 8, "blocks_per_cu");
     if (tile_elems)
         for (int g = 0; g < ctx->n_group; ++g)
@@ -2065,9 +2088,9 @@ dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings)
     // 2. the persistent grid: one LDS size for every tile, so the tiles are capped to an LDS budget
     //    (the per-group best tile, halved until it fits) and the budget and grid size are searched
     const size_t caps[] = { 24 << 10, 32 << 10, 40 << 10, 48 << 10, (size_t)kMaxTileLds };
-    std::vector<int32_t> best_tiles = pg;
+    std::vector<int32_t> best_tiles = pg, best_grouped = pg;
     int best_bpc = ctx->bpc, best_w = ctx->pwaves;
-    double best_us = 1e300;
+    double best_us = 1e300, best_grouped_us = 1e300;
     for (size_t cap : caps) {
         std::vector<int32_t> tl(ctx->n_group);
         bool fits = true;
@@ -2084,6 +2107,14 @@ dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings)
             ctx->groups[g].tile = tl[g];
         MIX_RUN(build_vtiles(ctx));
         a = make_args(ctx, ctx->u, nullptr, ctx->rhs, -1, 1, 0.0);
+        {
+            double us = 0.0;
+            MIX_RUN(time_launches(ctx, reps, [&]() { return launch_vol(ctx, a, DGX_MIXED_LAUNCH_GROUPED); }, &us));
+            if (us < best_grouped_us) {
+                best_grouped_us = us;
+                best_grouped = tl;
+            }
+        }
         const int max_bpc = std::max(1, std::min(8, (int)((160u << 10) / ctx->lds_max)));
         for (int w = 2; w <= 4; ++w) {
             ctx->pwaves = w;
@@ -2104,6 +2135,7 @@ dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings)
         }
     }
     ctx->tiles_of[DGX_MIXED_LAUNCH_PERSISTENT] = best_tiles;
+    ctx->tiles_of[DGX_MIXED_LAUNCH_GROUPED] = best_grouped;
     ctx->bpc = best_bpc;
     ctx->pwaves = best_w;
     for (int g = 0; g < ctx->n_group; ++g)

@@ -532,6 +532,38 @@ __device__ inline int64_t caller_index(int layout, int64_t e, int64_t v, int64_t
     return layout == 0 ? (e * NV + v) * NS + n : (v * E + e) * NS + n;
 }
 
+// The upload of a linearized product's input fused with its face traces: caller layout -> the internal
+// slot-order array (times 1/(w J) when imass is given, the adjoint's M^-1) and, from the same LDS tile,
+// the traces the linearized kernel reads for the neighbours.  One pass instead of upload + trace.
+template <int D, int P>
+__global__ __launch_bounds__(256) void upload_trace_kernel(const double* src, int layout, const int32_t* slot_elem,
+                                                           const double* imass, double* dst, double* tr, int64_t E,
+                                                           int64_t Ep)
+{
+    using S = Shape<D, P>;
+    constexpr int NS = S::NS, NV = S::NV, EPB = S::EPB;
+    __shared__ double s_u[EPB * NV * NS];
+    const int tid = threadIdx.x;
+    const int el = tid / NS;
+    const int n = tid - el * NS;
+    const bool in_tile = el < EPB;
+    const int64_t s = (int64_t)blockIdx.x * EPB + el;
+    const bool valid = in_tile && s < Ep;
+    double* su = s_u + (in_tile ? el : 0) * NV * NS;
+    if (valid) {
+        const int64_t e = slot_elem ? (int64_t)slot_elem[s] : s;
+        const double sc = imass ? imass[s * NS + n] : 1.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double x = e >= 0 ? src[caller_index(layout, e, v, n, E, NV, NS)] * sc : 0.0;
+            su[v * NS + n] = x;
+            dst[(v * Ep + s) * NS + n] = x;
+        }
+    }
+    __syncthreads();
+    write_traces<D, P>(su, tr, s, n, valid);
+}
+
 __global__ void to_internal_kernel(const double* src, int layout, double* dst, const int32_t* slot_elem, int64_t E,
                                    int64_t Ep, int NV, int NS)
 {
@@ -672,6 +704,24 @@ hipError_t upload_ops(int p, const Ops1D& ops)
 hipError_t launch_stage(int d, int p, int flux, const StageArgs& args, hipStream_t st)
 {
 #define DGX_CALL(DD, PP) launch_stage_dp<DD, PP>(args, flux, st)
+    DGX_DISPATCH(d, p, DGX_CALL);
+#undef DGX_CALL
+}
+
+template <int D, int P>
+static hipError_t launch_upload_trace_dp(const double* src, int layout, const int32_t* slot_elem, const double* imass,
+                                         double* dst, double* tr, int64_t E, int64_t Ep, hipStream_t st)
+{
+    const int EPB = Shape<D, P>::EPB;
+    const int64_t grid = (Ep + EPB - 1) / EPB;
+    upload_trace_kernel<D, P><<<dim3((unsigned)grid), dim3(256), 0, st>>>(src, layout, slot_elem, imass, dst, tr, E, Ep);
+    return hipGetLastError();
+}
+
+hipError_t launch_upload_trace(int d, int p, const double* src, int layout, const int32_t* slot_elem,
+                               const double* imass, double* dst, double* tr, int64_t E, int64_t Ep, hipStream_t st)
+{
+#define DGX_CALL(DD, PP) launch_upload_trace_dp<DD, PP>(src, layout, slot_elem, imass, dst, tr, E, Ep, st)
     DGX_DISPATCH(d, p, DGX_CALL);
 #undef DGX_CALL
 }

@@ -224,7 +224,8 @@ class MixedSolver:
         self._check(self._lib.dgx_mixed_step(self._ctx, float(dt), int(nsteps)))
 
     def set_launch(self, mode="persistent", tile_elems=None, blocks_per_cu=None):
-        m = {"persistent": _ffi.MIXED_LAUNCH_PERSISTENT, "per_group": _ffi.MIXED_LAUNCH_PER_GROUP}[mode]
+        m = {"persistent": _ffi.MIXED_LAUNCH_PERSISTENT, "per_group": _ffi.MIXED_LAUNCH_PER_GROUP,
+             "grouped": _ffi.MIXED_LAUNCH_GROUPED}[mode]
         cur_mode, tiles, bpc = self.get_launch()
         t = None
         if tile_elems is not None:
@@ -238,7 +239,9 @@ class MixedSolver:
         bpc = ctypes.c_int32()
         self._check(self._lib.dgx_mixed_get_launch(self._ctx, ctypes.byref(mode), _ffi.i32ptr(tiles),
                                                    ctypes.byref(bpc)))
-        return ("persistent" if mode.value == _ffi.MIXED_LAUNCH_PERSISTENT else "per_group"), tiles, bpc.value
+        names = {_ffi.MIXED_LAUNCH_PERSISTENT: "persistent", _ffi.MIXED_LAUNCH_PER_GROUP: "per_group",
+                 _ffi.MIXED_LAUNCH_GROUPED: "grouped"}
+        return names[mode.value], tiles, bpc.value
 
     def autotune(self, reps=5):
         """Runs the library's autotuner; returns the best microseconds per launch of every group."""

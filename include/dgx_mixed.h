@@ -30,8 +30,9 @@ extern "C" {
 #endif
 
 enum { DGX_ELEM_HEX = 0, DGX_ELEM_WEDGE = 1, DGX_ELEM_TET = 2, DGX_ELEM_PYR = 3 };
-/* Stage launch form: one persistent grid over all (type, p) tiles, or one launch per (type, p) group. */
-enum { DGX_MIXED_LAUNCH_PERSISTENT = 0, DGX_MIXED_LAUNCH_PER_GROUP = 1 };
+/* Stage launch form: one persistent grid over all (type, p) tiles (work counter), one launch per
+ * (type, p) group, or one grouped launch with a workgroup per tile of every group. */
+enum { DGX_MIXED_LAUNCH_PERSISTENT = 0, DGX_MIXED_LAUNCH_PER_GROUP = 1, DGX_MIXED_LAUNCH_GROUPED = 2 };
 
 /* Mesh, geometry and operator tables of a straight-sided (affine) mixed mesh.  Elements must be sorted
  * by group: group g holds elements group_start[g] .. group_start[g+1]-1. */
@@ -130,7 +131,8 @@ dgx_status dgx_mixed_get_launch(const dgx_mixed_ctx* ctx, int* mode, int32_t* ti
  * and keeps the fastest for the per-group form; for the persistent form, whose one launch has one LDS
  * size, caps those tiles to an LDS budget (24..60 KB, halving a tile until it fits) and times the whole
  * persistent volume launch for every budget, every blocks_per_cu the budget lets fit on a CU and
- * every occupancy variant of the kernel (dgx_mixed_persistent_waves), keeping the fastest.  Runs the residual form of the kernels: the state is untouched.  timings
+ * every occupancy variant of the kernel (dgx_mixed_persistent_waves), keeping the fastest; the grouped
+ * form takes the budget whose one launch is fastest.  Runs the residual form of the kernels: the state is untouched.  timings
  * (nullable) receives per group the best microseconds per launch. */
 dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings);
 /* The occupancy variant of the persistent volume kernel the autotuner chose: 2, 3 or 4 waves per SIMD

@@ -4,8 +4,8 @@ Small seeded mixed meshes (HEX, WEDGE, TET, PYR, per-element p in 1..5) with a p
 state:
 - the residual, before and after M^-1, and 3 SSP-RK3 steps equal the oracle to 1e-12 relative (L2) and
   1e-11 (max-norm), for Lax-Friedrichs and Roe-Pike;
-- the persistent launch, the per-group launches and every tile shape the autotuner picks give
-  bit-identical results (each lane's sums do not depend on the launch shape);
+- the persistent launch, the grouped launch, the per-group launches and every tile shape the autotuner
+  picks give bit-identical results (each lane's sums do not depend on the launch shape);
 - repeated graph replays equal single steps;
 - the sum-factorized volume paths (HEX and WEDGE tensor products with face-modal HEX lifts and traces,
   TET and PYR collapsed coordinates) equal the dense path (and the oracle) to 1e-12.
@@ -76,7 +76,7 @@ def test_launch_forms_and_tiles_bit_identical():
     s.step(dt, 2)
     u0 = s.get_state()
     _, tiles0, _ = s.get_launch()
-    for mode in ("per_group", "persistent"):
+    for mode in ("per_group", "persistent", "grouped"):
         for scale in (1, 2, 4):
             tiles = np.maximum(tiles0 // scale, 1)
             for bpc in (1, 3):
