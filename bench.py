@@ -156,6 +156,47 @@ def precision_form(args, dg, torch, mesh, geo, u0, dt, fp64_solver, fp64_stage_m
             "meets_target": rel(r_mixed, r64) <= 1e-9}
 
 
+def ns_precision_form(args, dg, torch, mesh, geo, u0, dt, viscous, fp64_solver, fp64_stage_ms, fp64_ke):
+    """The Navier-Stokes HEX p=4 precision form (line-differenced FP32 hi/lo contractions of the own face
+    traces, the volume gradient and the weak divergence) against the FP64 run: stage time ratio, residual
+    error at the initial state, and over the KE_STEPS-step protocol the kinetic-energy history's largest
+    relative deviation and the final state error, next to FP64's own residual round-off floor."""
+    dev = torch.device("cuda:0")
+    ut = torch.from_numpy(u0).to(dev)
+    mixed = dg.Solver(mesh, geo, 4, num_flux="roe", viscous=viscous, precision="mixed")
+    mixed.set_state(ut)
+    if args.warmup > 0:
+        mixed.step(dt, args.warmup)
+    mixed.step(dt, args.steps)
+    mixed_stage_ms = mixed.last_elapsed_ms / (3 * args.steps)
+    mixed.set_state(ut)
+    r_mixed = torch.empty_like(ut)
+    mixed.compute_rhs(precision="mixed", out=r_mixed)
+    ke_m = ke_history(dg, torch, mixed, u0, dt, KE_STEPS, 10)
+    u_mixed = mixed.get_state(torch.empty_like(ut))
+    mixed.close()
+    fp64_solver.set_state(ut)
+    fp64_solver.step(dt, KE_STEPS)
+    u_fp64 = fp64_solver.get_state(torch.empty_like(ut))
+    fp64_solver.set_state(ut)
+    r64 = torch.empty_like(ut)
+    fp64_solver.compute_rhs(out=r64)
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    fp64_solver.set_state(ut * (1 + 1.1e-16 * torch.randn(ut.shape, generator=gen, dtype=torch.float64).to(dev)))
+    r_eps = torch.empty_like(ut)
+    fp64_solver.compute_rhs(out=r_eps)
+
+    def rel(a, b):
+        return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+    ke_dev = max(abs(a - b) / abs(b) for a, b in zip(ke_m["history"], fp64_ke["history"]))
+    return {"kernel": "dgx::ns4_stage_kernel<1,true>", "speedup_vs_fp64": fp64_stage_ms / mixed_stage_ms,
+            "stage_us": mixed_stage_ms * 1e3, "residual_rel_l2": rel(r_mixed, r64),
+            "state_rel_l2": rel(u_mixed, u_fp64), "state_steps": KE_STEPS,
+            "kinetic_energy_max_rel_dev": ke_dev, "kinetic_energy_history": ke_m["history"],
+            "fp64_roundoff_floor_rel_l2": rel(r_eps, r64), "target_rel_err": 1e-9}
+
+
 def fp64_vs_oracle(args, dg, torch, mesh, geo, u0, dt, p, num_flux, solver, viscous=None, steps=0, minv=False,
                    pre_minv=True):
     """GPU FP64 against the oracle at the benchmark's full size: the pre-M^-1 residual (pre_minv) and the
@@ -733,6 +774,9 @@ def main():
                                                viscous=viscous, minv=True, pre_minv=False)
     if viscous:
         out["kinetic_energy"] = ke_history(dg, torch, solver, u0, dt, KE_STEPS, 10)
+        if args.precision == "fp64" and not args.no_precision_form:
+            out["precision_form"] = ns_precision_form(args, dg, torch, mesh, geo, u0, dt, viscous, solver,
+                                                      ms / stages, out["kinetic_energy"])
         if args.oracle_check:
             out["small_mesh_vs_oracle"] = ns_small_vs_oracle(dg, KE_STEPS)
     if not args.no_cpu_baseline and not viscous:

@@ -28,6 +28,7 @@ struct dgx_ctx {
     double* lam = nullptr;       // NS: [Ep][nface][nf] lift trace factors
     double* fnv = nullptr;       // NS HEX p=4: [Ep][nface][3][nf] |J_f| n, own outward normal
     bool hex3 = false;           // HEX p=3 group kernel
+    bool ns4 = false;            // Navier-Stokes HEX p=4 kernel
     int64_t E = 0, F = 0;
     int64_t Ep = 0;              // number of slots (E plus padding)
     int64_t nmax = 0;            // entries of blockmax
@@ -216,6 +217,7 @@ hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin,
         a.mu = c->mu;
         a.Pr = c->Pr;
         a.eta = c->eta;
+        a.mixed = precision == DGX_PREC_MIXED ? 1 : 0;
         return dgx::launch_ns_stage(c->d, c->p, c->flux, a, c->gtr[0], c->stream);
     }
     return dgx::launch_stage(c->d, c->p, c->flux, a, c->stream);
@@ -439,9 +441,11 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     }
     // the HEX p=3 group kernel assumes aligned faces; other meshes run the generic kernel of that order
     c->hex3 = (d == 3 && p == 3 && !visc && aligned);
-    if (cfg->precision == DGX_PREC_MIXED && !c->hex3) {
+    // the precision form is built for Euler HEX p=3 (group kernel) and Navier-Stokes HEX p=4
+    c->ns4 = (d == 3 && p == 4 && visc);
+    if (cfg->precision == DGX_PREC_MIXED && !c->hex3 && !c->ns4) {
         delete c;
-        return DGX_ERR_INVALID_ARG;   // the precision form is built for Euler HEX p=3
+        return DGX_ERR_INVALID_ARG;
     }
 
     std::vector<int32_t> slot_elem;
@@ -723,8 +727,9 @@ dgx_status dgx_rhs(dgx_ctx* ctx, int precision, int apply_minv, int layout, int 
         return DGX_ERR_INVALID_ARG;
     if (precision != DGX_PREC_FP64 && precision != DGX_PREC_MIXED)
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: bad precision");
-    if (precision == DGX_PREC_MIXED && !ctx->hex3)
-        return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: the precision form is built for Euler HEX p=3");
+    if (precision == DGX_PREC_MIXED && !ctx->hex3 && !ctx->ns4)
+        return fail(ctx, DGX_ERR_INVALID_ARG,
+                    "dgx_rhs: the precision form is built for Euler HEX p=3 and Navier-Stokes HEX p=4");
     if (rhs && ((layout != DGX_LAYOUT_REF && layout != DGX_LAYOUT_SOA) ||
                 (ptr_kind != DGX_PTR_HOST && ptr_kind != DGX_PTR_DEVICE)))
         return fail(ctx, DGX_ERR_INVALID_ARG, "dgx_rhs: bad layout");

@@ -360,7 +360,31 @@ struct StageArgs {
     const double* fnv;      // HEX p=4: [E][NFACE][3][NF] |J_f| n of the own outward normal (set-up)
     double* gtr_out;        // HEX p=4: the n.f2 traces of the state a stage writes
     double mu, Pr, eta;     // viscosity, Prandtl number, BR2 penalty
+    int mixed;              // HEX p=4 Navier-Stokes: precision form of the residual's contractions
 };
+
+// sum_k c[k] t[base + k st] (k < N) in the precision form: the line's first value t0 goes through FP64
+// with the FP64 sum of the coefficients, the FP64 differences t_k - t0 are contracted as FP32 hi/lo parts
+// with FP32 coefficients and FP32 accumulation of each part, the two partial sums added in FP64.  The
+// FP32 round-off then scales with the variation of the data along the line, not with its size.
+template <int N>
+__device__ inline double dot_diff_hl(const double* c, const double* t, int base, int st)
+{
+    const double t0 = t[base];
+    double csum = c[0];
+    float hi = 0.0f, lo = 0.0f;
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+        const double d = t[base + k * st] - t0;
+        const float h = (float)d;
+        const float l = (float)(d - (double)h);
+        const float ck = (float)c[k];
+        hi = fmaf(ck, h, hi);
+        lo = fmaf(ck, l, lo);
+        csum += c[k];
+    }
+    return csum * t0 + ((double)hi + (double)lo);
+}
 
 // Second-order (viscous) part of the Navier-Stokes flux at one point, negated as the reference stores
 // it (flux_navier_stokes_T.c:1005-1040): f2[i + D*eq] = -F_v(i, eq).  g[v + NV*j] = d(var v)/dx_j.

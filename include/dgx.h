@@ -35,11 +35,12 @@ typedef enum dgx_status {
 
 enum { DGX_PDE_EULER = 0, DGX_PDE_NAVIER_STOKES = 1 };   /* test_case.c pde_index: euler, navier_stokes */
 enum { DGX_FLUX_LAX_FRIEDRICHS = 0, DGX_FLUX_ROE_PIKE = 1 };   /* numerical_flux_euler_T.c:60-327, 470-778 */
-/* DGX_PREC_FP64: the plain form.  DGX_PREC_MIXED: the face traces and the weak derivative contract, along
- * every 4-node line, the FP64 differences from the line's first node as FP32 hi/lo parts with FP32
- * accumulation (the line's constant part in FP64); the face lift and all else stay FP64 (Euler HEX p=3
- * on aligned meshes only; dgx_create and dgx_rhs return DGX_ERR_INVALID_ARG elsewhere).  No reference
- * counterpart: the reference is FP64. */
+/* DGX_PREC_FP64: the plain form.  DGX_PREC_MIXED: the residual's operator contractions (face traces, weak
+ * derivative, and for Navier-Stokes the volume gradient) contract, along every line, the FP64 differences
+ * from the line's first node as FP32 hi/lo parts with FP32 accumulation (the line's constant part in
+ * FP64); the face lift and all else stay FP64 (Euler HEX p=3 on aligned meshes and Navier-Stokes HEX p=4;
+ * dgx_create and dgx_rhs return DGX_ERR_INVALID_ARG elsewhere).  No reference counterpart: the reference
+ * is FP64. */
 enum { DGX_PREC_FP64 = 0, DGX_PREC_MIXED = 1 };
 enum { DGX_LAYOUT_REF = 0, DGX_LAYOUT_SOA = 1 };               /* SOA = device layout u[v][e][n] */
 /* Explicit time stepper, the reference's test_case->solver_type_e (solve_explicit.c:150-170):

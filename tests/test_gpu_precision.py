@@ -10,7 +10,9 @@ runs and how far it is from the FP64 path:
   residual (measured on these 8^3 states: 4.4e-6 at M = 0.1, 4.0e-6 at M = 0.3; 9.4e-5 on the 64^3
   benchmark state; FP32 sums of the undifferenced values gave 3e-2 there);
 - it never touches the FP64 path: FP64 results are bit-identical whether or not a mixed context ran;
-- it is refused where it is not built (other orders, Navier-Stokes).
+- the Navier-Stokes HEX p=4 form (own face traces, volume gradient and weak divergence differenced the
+  same way) stays close to its FP64 run on the Re = 1600 Taylor-Green state;
+- it is refused where it is not built (other orders, Navier-Stokes at p != 4).
 """
 import math
 
@@ -70,6 +72,36 @@ def test_mixed_steps_stay_close():
     err = np.linalg.norm(ub - ua) / np.linalg.norm(ua)
     print(f"state rel L2 after 10 steps {err:.3e}", flush=True)
     assert 0 < err < 1e-9
+    a.close()
+    b.close()
+
+
+def test_ns_p4_mixed_close_to_fp64():
+    """Navier-Stokes HEX p=4 precision form on the 6^3 Taylor-Green state at Re = 1600, M = 0.1: residual
+    error against FP64 and the state error after 10 SSP-RK3 steps."""
+    _require_gpu()
+    print("case: ns p=4 precision form 6^3", flush=True)
+    visc = (1.0 / 1600.0, 0.71)
+    L = 2 * math.pi
+    mesh = dg.structured_mesh(3, (6, 6, 6), (0, 0, 0), (L, L, L), p_g=1)
+    geo = dg.geometry(mesh, 4)
+    u = dg.ic_taylor_green(mesh, 4, 0.1)
+    a = dg.Solver(mesh, geo, 4, num_flux="roe", viscous=visc)
+    b = dg.Solver(mesh, geo, 4, num_flux="roe", viscous=visc, precision="mixed")
+    a.set_state(u)
+    b.set_state(u)
+    r64, _ = a.compute_rhs()
+    rmx, _ = b.compute_rhs(precision="mixed")
+    err = float(np.linalg.norm(rmx - r64) / np.linalg.norm(r64))
+    print(f"ns p=4 mixed vs fp64 residual rel L2 {err:.3e}", flush=True)
+    dt = dg.dt_cfl(mesh, 4, u)
+    a.step(dt, 10)
+    b.step(dt, 10)
+    ua, ub = a.get_state(), b.get_state()
+    es = float(np.linalg.norm(ub - ua) / np.linalg.norm(ua))
+    print(f"ns p=4 mixed vs fp64 state rel L2 after 10 steps {es:.3e}", flush=True)
+    assert 0 < err < 1e-3
+    assert 0 < es < 1e-8
     a.close()
     b.close()
 
