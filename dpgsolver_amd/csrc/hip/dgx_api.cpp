@@ -44,8 +44,7 @@ struct dgx_ctx {
     uint8_t* nord = nullptr;     // [Ep][nface] face node correspondence case (null when every face is aligned)
     int32_t* slot_elem = nullptr;   // [Ep]
     int32_t* elem_slot = nullptr;   // [E]
-    uint8_t* face_list = nullptr;   // [G][48]
-    uint8_t* face_count = nullptr;  // [G]
+    uint64_t* face_item = nullptr;  // [G][48] HEX p=3 group face items (dgx::Hex3Args::face_item)
     double* lin_x = nullptr;     // [nv][Ep][ns] tangent / cotangent of dgx_jvp / dgx_vjp (allocated on first use)
     double* lin_tr = nullptr;    // [Ep][nface][nv][nf] its traces
     double* blockmax = nullptr;  // [nmax]
@@ -87,7 +86,7 @@ void free_all(dgx_ctx* c)
         c->graph[i] = nullptr;
     }
     void* ptrs[] = { c->u, c->up, c->rhs, c->stage_buf, c->met, c->imass, c->tr[0], c->tr[1], c->nbr,
-                     c->slot_elem, c->elem_slot, c->face_list, c->face_count, c->blockmax, c->dmax,
+                     c->slot_elem, c->elem_slot, c->face_item, c->blockmax, c->dmax,
                      c->bad_elem, c->nonfinite, c->gate, c->lin_x, c->lin_tr, c->gtr[0], c->gtr[1], c->lam,
                      c->fnv, c->nord };
     for (void* p : ptrs)
@@ -177,8 +176,7 @@ hipError_t launch_any(dgx_ctx* c, int stage, double dt, int apply_minv, int tin,
         a.tr_out = c->tr[tin ^ 1];
         a.nbr = c->nbr;
         a.slot_elem = c->slot_elem;
-        a.face_list = c->face_list;
-        a.face_count = c->face_count;
+        a.face_item = c->face_item;
         a.rhs_out = c->rhs;
         a.blockmax = c->blockmax;
         a.bad_elem = c->bad_elem;
@@ -474,11 +472,10 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
             nbr[s * nface + f] = dgx::pack_nbr((uint32_t)elem_slot[nbe[k]], (uint32_t)nbl[k], (uint32_t)nbr_right[k]);
         }
     }
-    std::vector<uint8_t> flist, fcount;
+    std::vector<uint64_t> fitem;
     if (c->hex3) {
         const int64_t G = Ep / 8;
-        flist.assign((size_t)G * 48, 0);
-        fcount.assign((size_t)G, 0);
+        fitem.assign((size_t)G * 48, ~0ull);
         for (int64_t g = 0; g < G; ++g) {
             int cnt = 0;
             for (int sl = 0; sl < 8; ++sl) {
@@ -491,10 +488,9 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
                     const bool right = (rec >> 31) != 0;
                     const bool local = (nsl / 8) == g;
                     if (!local || !right)
-                        flist[g * 48 + cnt++] = (uint8_t)(sl * 6 + f);
+                        fitem[g * 48 + cnt++] = ((uint64_t)(sl * 6 + f) << 32) | rec;
                 }
             }
-            fcount[g] = (uint8_t)cnt;
         }
     }
 
@@ -591,8 +587,7 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
     if (!aligned)
         step(hipMalloc(&c->nord, (size_t)Ep * nface), "hipMalloc(nord)");
     if (c->hex3) {
-        step(hipMalloc(&c->face_list, flist.size()), "hipMalloc(face_list)");
-        step(hipMalloc(&c->face_count, fcount.size()), "hipMalloc(face_count)");
+        step(hipMalloc(&c->face_item, sizeof(uint64_t) * fitem.size()), "hipMalloc(face_item)");
     }
     if (visc) {
         step(hipMalloc(&c->gtr[0], gbytes), "hipMalloc(gtr)");
@@ -625,8 +620,8 @@ dgx_status dgx_create(const dgx_config* cfg, const dgx_mesh* mesh, const dgx_geo
             step(hipMemcpy(c->nord, nord.data(), nord.size(), hipMemcpyHostToDevice), "copy nord");
         }
         if (c->hex3) {
-            step(hipMemcpy(c->face_list, flist.data(), flist.size(), hipMemcpyHostToDevice), "copy face_list");
-            step(hipMemcpy(c->face_count, fcount.data(), fcount.size(), hipMemcpyHostToDevice), "copy face_count");
+            step(hipMemcpy(c->face_item, fitem.data(), sizeof(uint64_t) * fitem.size(), hipMemcpyHostToDevice),
+                 "copy face_item");
         }
         step(hipMemset(c->u, 0, sbytes), "memset u");
         step(hipMemset(c->up, 0, sbytes), "memset up");

@@ -491,8 +491,8 @@ struct Hex3Args {
     double* tr_out;             // [Ep][6][NV][16] (faces read by another group only)
     const uint32_t* nbr;        // [Ep][6] packed neighbour records (slot of the neighbour)
     const int32_t* slot_elem;   // [Ep] element of the slot, -1 for padding
-    const uint8_t* face_list;   // [G][48] faces (slot*6 + f) whose flux the group evaluates
-    const uint8_t* face_count;  // [G]
+    const uint64_t* face_item;  // [G][48] faces whose flux the group evaluates: (slot-in-group*6 + f) << 32 | the
+                                // face's neighbour record; unused entries all ones, used ones first
     double* rhs_out;            // [NV][Ep][64] residual mode output
     double* blockmax;           // [Ep] max |rhs| per element
     unsigned long long* bad_elem;
