@@ -47,6 +47,40 @@ struct GroupDev {
 
 constexpr int kTpLds = 96;   // doubles at the start of a volume tile's LDS: the 1-D factors
 
+// Modes of a (type, p) element of the sum-factorized paths.
+__host__ __device__ constexpr int sf_modes(int type, int p)
+{
+    const int n = p + 1;
+    return type == DGX_ELEM_HEX ? n * n * n
+           : type == DGX_ELEM_WEDGE ? n * n * (n + 1) / 2
+           : type == DGX_ELEM_TET ? n * (n + 1) * (n + 2) / 6 : n * (n + 1) * (2 * n + 1) / 6;
+}
+
+// Doubles of a sum-factorized group's factor block: HEX B1, D1 [n][n], w1, L(-1), L(+1) [n]; WEDGE also
+// Bt, Gr, Gs [n^2][Nt], wt [n^2], the TRI edge matrices E [3][n][Nt] and tm, tp [n]; TET / PYR: A, Ad
+// [n][n]; B, Bd; C, Cd (TET: only the entries of its modes, [Ns][n]); xa, xb, xc [n]; node weights [n^3]; mode
+// scales [Ns].
+__host__ __device__ constexpr int sf_tab_len(int type, int p)
+{
+    const int n = p + 1, nt = n * (n + 1) / 2;
+    if (type == DGX_ELEM_TET || type == DGX_ELEM_PYR) {
+        const int sb = type == DGX_ELEM_TET ? n * n * n : n * n;
+        const int sc = type == DGX_ELEM_TET ? sf_modes(type, p) * n : n * n * n;
+        return 2 * n * n + 2 * sb + 2 * sc + 3 * n + n * n * n + sf_modes(type, p);
+    }
+    return type == DGX_ELEM_WEDGE ? 2 * n * n + n + 3 * n * n * nt + n * n + 3 * n * nt + 2 * n
+                                  : 2 * n * n + 3 * n;
+}
+
+// LDS doubles ahead of a tile's element slots: the whole factor block (the TRI and collapsed-coordinate
+// tables are read at lane-dependent rows, which from global memory cost one cache line per lane), at
+// least kTpLds, in whole 128-byte lines.
+__host__ __device__ constexpr int sf_tab_lds(int type, int p)
+{
+    const int r = (sf_tab_len(type, p) + 15) / 16 * 16;
+    return r > kTpLds ? r : kTpLds;
+}
+
 // LDS doubles of one HEX element slot: 15 n^3 (nodal fluxes) and at least 5 n^3 + 30 n^2 (mode work +
 // face-modal work of the 6 faces)
 __host__ __device__ constexpr int hex_slot(int n)
@@ -323,12 +357,16 @@ __device__ __attribute__((always_inline)) inline void vol_tile_hex(const MixArgs
     const bool act = slot < t.count;
     const int64_t e = t.e0 + slot;
     constexpr int per = hex_slot(n);
-    double* B1 = lds;
-    double* D1 = lds + n2;
+    // B1 and D1 are read at compile-time indices (the same entry on every lane) through the constant
+    // address space: scalar loads and scalar FMA operands, where an LDS copy costs an LDS read per
+    // coefficient in passes that are already bound by their LDS line traffic
+    using cdp = const __attribute__((address_space(4))) double*;
+    const cdp B1 = (cdp)(a.ops + G.tp_off);
+    const cdp D1 = B1 + n2;
     double* w1 = lds + 2 * n2;
     double* tm1 = lds + 2 * n2 + n;        // L_i(-1), L_i(+1): the traces of the 1-D basis
     double* tp1 = lds + 2 * n2 + 2 * n;
-    double* W = lds + kTpLds + slot * per;
+    double* W = lds + sf_tab_lds(DGX_ELEM_HEX, P) + slot * per;
     double* S = W + 5 * n3;                // face-modal work [face][v][n^2]
     if (threadIdx.x < 2 * n2 + 3 * n)
         lds[threadIdx.x] = a.ops[G.tp_off + threadIdx.x];
@@ -389,11 +427,29 @@ __device__ __attribute__((always_inline)) inline void vol_tile_hex(const MixArgs
             }
         }
         __syncthreads();
-        // x: lines (d, v, qy, qz)
+        // x: lines (d, v, qy, qz): D1^T for d = 0, then B1^T for d = 1, 2 (one operator per loop, so that
+        // every lane reads the same coefficient)
         if (act) {
             for (int ln = l; ln < 15 * n2; ln += tpe) {
                 const int dv = ln / n2, rr = ln - dv * n2;
-                const double* op = dv < 5 ? D1 : B1;
+                const int base = dv * n3 + rr * n;
+                if (ln >= 5 * n2)
+                    break;
+                double in[n];
+#pragma unroll
+                for (int q = 0; q < n; ++q)
+                    in[q] = W[base + q];
+#pragma unroll
+                for (int i = 0; i < n; ++i) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < n; ++q)
+                        acc = fma(D1[q * n + i], in[q], acc);
+                    W[base + i] = acc;
+                }
+            }
+            for (int ln = 5 * n2 + l; ln < 15 * n2; ln += tpe) {
+                const int dv = ln / n2, rr = ln - dv * n2;
                 const int base = dv * n3 + rr * n;
                 double in[n];
 #pragma unroll
@@ -404,7 +460,7 @@ __device__ __attribute__((always_inline)) inline void vol_tile_hex(const MixArgs
                     double acc = 0.0;
 #pragma unroll
                     for (int q = 0; q < n; ++q)
-                        acc = fma(op[q * n + i], in[q], acc);
+                        acc = fma(B1[q * n + i], in[q], acc);
                     W[base + i] = acc;
                 }
             }
@@ -589,10 +645,11 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
     const int slot = threadIdx.x / tpe, l = threadIdx.x - slot * tpe;
     const bool act = slot < t.count;
     const int64_t e = t.e0 + slot;
-    double* B1 = lds;
-    double* D1 = lds + n2;
-    double* w1 = lds + 2 * n2;
-    const double* tri = a.ops + G.tp_off + 2 * n2 + n;   // Bt, Gr, Gs [n2][nt], wt [n2]
+    constexpr int TAB = sf_tab_lds(DGX_ELEM_WEDGE, P);
+    const double* B1 = lds;
+    const double* D1 = lds + n2;
+    const double* w1 = lds + 2 * n2;
+    const double* tri = lds + 2 * n2 + n;                  // Bt, Gr, Gs [n2][nt], wt [n2]
     const double* Bt = tri;
     const double* Gr = tri + n2 * nt;
     const double* Gs = tri + 2 * n2 * nt;
@@ -600,11 +657,11 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
     const double* Ew = wt + n2;                             // E [3][n][Nt]: the TRI edge matrices
     const double* tmw = Ew + 3 * n * nt;                    // L_k(-1), L_k(+1)
     const double* tpw = tmw + n;
-    double* W = lds + kTpLds + slot * (15 * n3 + 5 * ns);
+    double* W = lds + TAB + slot * (15 * n3 + 5 * ns);
     double* Um = W + 15 * n3;                               // modal work [v][k][m]
     double* S = W;                                          // face-modal work [face][v][n^2] (25 n^2 <= 15 n^3)
-    if (threadIdx.x < 2 * n2 + n)
-        lds[threadIdx.x] = a.ops[G.tp_off + threadIdx.x];
+    for (int i = threadIdx.x; i < sf_tab_len(DGX_ELEM_WEDGE, P); i += kBlock)
+        lds[i] = a.ops[G.tp_off + i];
     if (act) {
         const double* ue = a.u + a.dof[e];
         for (int i = l; i < 5 * ns; i += tpe)
@@ -862,7 +919,8 @@ struct Collapsed {
     }
     __host__ __device__ static constexpr int n_modes() { return off(n, 0); }
     __host__ __device__ static constexpr int size_b() { return TYPE == DGX_ELEM_TET ? n * n * n : n * n; }
-    __host__ __device__ static constexpr int size_c() { return TYPE == DGX_ELEM_TET ? n * n * n * n : n * n * n; }
+    // TET: C(i, j)[r][k] only for the modes (k <= kmax(i, j)), at ((first mode of (i, j)) + k) n + r
+    __host__ __device__ static constexpr int size_c() { return TYPE == DGX_ELEM_TET ? n_modes() * n : n * n * n; }
     // (i, j) of the pair-th valid pair
     __device__ static void pair(int pr, int& i, int& j)
     {
@@ -874,9 +932,10 @@ struct Collapsed {
         j = pr;
     }
     __device__ static int bidx(int i, int q, int j) { return TYPE == DGX_ELEM_TET ? (i * n + q) * n + j : q * n + j; }
-    __device__ static int cidx(int i, int j, int r, int k)
+    // o = off(i, j), k <= kmax(i, j)
+    __device__ static int cidx(int i, int j, int o, int r, int k)
     {
-        return TYPE == DGX_ELEM_TET ? ((i * n + j) * n + r) * n + k : (((i > j ? i : j) * n + r) * n + k);
+        return TYPE == DGX_ELEM_TET ? (o + k) * n + r : (((i > j ? i : j) * n + r) * n + k);
     }
 };
 
@@ -898,9 +957,10 @@ __device__ __attribute__((always_inline)) inline void vol_tile_collapsed(const M
     const int slot = threadIdx.x / tpe, l = threadIdx.x - slot * tpe;
     const bool act = slot < t.count;
     const int64_t e = t.e0 + slot;
-    const double* tab = a.ops + G.tp_off;
-    double* Al = lds;             // A, Ad [n][n] staged in LDS
-    double* Adl = lds + n2;
+    constexpr int TAB = sf_tab_lds(TYPE, P);
+    const double* tab = lds;      // the factor block, staged in LDS
+    const double* Al = tab;       // A, Ad [n][n]
+    const double* Adl = tab + n2;
     const double* Bt = tab + 2 * n2;
     const double* Bdt = Bt + K::size_b();
     const double* Ct = Bdt + K::size_b();
@@ -910,10 +970,10 @@ __device__ __attribute__((always_inline)) inline void vol_tile_collapsed(const M
     const double* xc = xb + n;
     const double* wn = xc + n;
     const double* sm = wn + n3;
-    double* W = lds + kTpLds + slot * (15 * n3 + 5 * ns);
+    double* W = lds + TAB + slot * (15 * n3 + 5 * ns);
     double* Um = W + 15 * n3;
-    if (threadIdx.x < 2 * n2)
-        lds[threadIdx.x] = tab[threadIdx.x];
+    for (int i = threadIdx.x; i < sf_tab_len(TYPE, P); i += kBlock)
+        lds[i] = a.ops[G.tp_off + i];
     if (act) {
         const double* ue = a.u + a.dof[e];
         for (int i = l; i < 5 * ns; i += tpe)
@@ -937,7 +997,7 @@ __device__ __attribute__((always_inline)) inline void vol_tile_collapsed(const M
                     double acc = 0.0;
 #pragma unroll
                     for (int k = 0; k < n; ++k)
-                        acc = fma(Ct[K::cidx(i, j, r, k)], in[k], acc);
+                        acc = fma(Ct[K::cidx(i, j, o, r, k <= km ? k : km)], in[k], acc);   // in[k > km] = 0
                     W[v * n3 + i + n * j + n2 * r] = acc;
                 }
             }
@@ -1114,8 +1174,8 @@ __device__ __attribute__((always_inline)) inline void vol_tile_collapsed(const M
                     double acc = 0.0;
 #pragma unroll
                     for (int r = 0; r < n; ++r) {
-                        acc = fma(Ct[K::cidx(i, j, r, k)], z0[r], acc);
-                        acc = fma(Cdt[K::cidx(i, j, r, k)], z2[r], acc);
+                        acc = fma(Ct[K::cidx(i, j, o, r, k)], z0[r], acc);
+                        acc = fma(Cdt[K::cidx(i, j, o, r, k)], z2[r], acc);
                     }
                     Um[v * ns + o + k] = sm[o + k] * acc;
                 }
@@ -1355,26 +1415,21 @@ dgx_status fail(dgx_mixed_ctx* c, dgx_status s, const std::string& m)
 size_t tile_lds(const GroupDev& g, int tile)
 {
     size_t per = 5 * (size_t)g.ns + 15 * (size_t)g.nc;      // dense: coefficients + fluxes
-    if (g.tp_off >= 0 && g.type == DGX_ELEM_HEX)
-        per = (size_t)hex_slot(g.p + 1);                     // in place, n^3 nodes = modes
-    else if (g.tp_off >= 0)
-        per = 15 * (size_t)g.nc + 5 * (size_t)g.ns;          // WEDGE: nodal + modal work
-    return (kTpLds + (size_t)tile * per) * sizeof(double);
+    size_t tab = kTpLds;
+    if (g.tp_off >= 0) {
+        tab = (size_t)sf_tab_lds(g.type, g.p);               // the staged factor block
+        if (g.type == DGX_ELEM_HEX)
+            per = (size_t)hex_slot(g.p + 1);                 // in place, n^3 nodes = modes
+        else
+            per = 15 * (size_t)g.nc + 5 * (size_t)g.ns;      // nodal + modal work
+    }
+    return (tab + (size_t)tile * per) * sizeof(double);
 }
 
-// doubles of a group's factor block: B1, D1 [n][n], w1 [n]; WEDGE also Bt, Gr, Gs [n^2][Nt], wt [n^2]
+// doubles of a group's factor block (sf_tab_len)
 int64_t tp_len(const GroupDev& g)
 {
-    const int64_t n = g.p + 1, nt = n * (n + 1) / 2;
-    if (g.type == DGX_ELEM_TET || g.type == DGX_ELEM_PYR) {
-        // A, Ad; B, Bd; C, Cd; xa, xb, xc; node weights; mode scales
-        const int64_t sb = g.type == DGX_ELEM_TET ? n * n * n : n * n;
-        const int64_t sc = g.type == DGX_ELEM_TET ? n * n * n * n : n * n * n;
-        return 2 * n * n + 2 * sb + 2 * sc + 3 * n + n * n * n + g.ns;
-    }
-    // WEDGE: B1, D1, w1, Bt, Gr, Gs, wt, then the edge matrices E [3][n][Nt] and tm, tp [n]
-    return g.type == DGX_ELEM_WEDGE ? 2 * n * n + n + 3 * n * n * nt + n * n + 3 * n * nt + 2 * n
-                                    : 2 * n * n + 3 * n;
+    return sf_tab_len(g.type, g.p);
 }
 
 bool tile_ok(const GroupDev& g, int tile)

@@ -422,6 +422,18 @@ def collapsed_factors(etype, p):
     return A, Ad, B, Bd, C, Cd, xa, xb, xc, w, s
 
 
+def collapsed_tables(etype, p):
+    """The factor block of a TET / PYR group as the sum-factorized kernels read it: collapsed_factors
+    flattened in order, TET's C and Cd kept only at its modes, row (mode (i, j, k) in mode_list order)
+    times n + r."""
+    A, Ad, B, Bd, C, Cd, xa, xb, xc, w, s = collapsed_factors(etype, p)
+    if etype == TET:
+        modes = mode_list(etype, p)
+        C = np.array([C[i, j, :, k] for (i, j, k) in modes])
+        Cd = np.array([Cd[i, j, :, k] for (i, j, k) in modes])
+    return np.concatenate([x.ravel() for x in (A, Ad, B, Bd, C, Cd, xa, xb, xc, w, s)])
+
+
 def collapsed_dense(etype, p):
     """The dense volume operators (B [Nc][Ns], G [3][Nc][Ns]) rebuilt from collapsed_factors by the
     chain rule of the collapse - the kernels' sum-factorized TET / PYR paths apply exactly this."""

@@ -57,7 +57,7 @@ def build_tables(mesh, sum_factorized=True):
         vol.append(blk)
         pos += blk.size
         if t in (M.TET, M.PYR) and sum_factorized:
-            tp = np.concatenate([x.ravel() for x in M.collapsed_factors(t, p)])
+            tp = M.collapsed_tables(t, p)
             vol_tp_off.append(pos)
             vol.append(tp)
             pos += tp.size

@@ -207,3 +207,25 @@ def test_collapsed_factors(etype, p):
     assert np.abs(Bc - B).max() < 1e-12
     assert np.abs(Gc - G).max() < 1e-10 * max(1.0, np.abs(G).max())
     assert np.abs(M.collapsed_factors(etype, p)[9] - w).max() < 1e-15
+
+
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_collapsed_tables_tet_compact(p):
+    """The TET factor block keeps C, Cd only at the modes (row = mode in mode_list order, then r): the
+    padded entries it drops are zero, and the block's length is the kernels' (2n^2 + 2n^3 + 2 Ns n + 3n +
+    n^3 + Ns)."""
+    n = p + 1
+    A, Ad, B, Bd, C, Cd, xa, xb, xc, w, s = M.collapsed_factors(M.TET, p)
+    tab = M.collapsed_tables(M.TET, p)
+    modes = M.mode_list(M.TET, p)
+    ns = len(modes)
+    assert tab.size == 2 * n * n + 2 * n ** 3 + 2 * ns * n + 3 * n + n ** 3 + ns
+    o = 2 * n * n + 2 * n ** 3
+    Cc = tab[o:o + ns * n].reshape(ns, n)
+    Cdc = tab[o + ns * n:o + 2 * ns * n].reshape(ns, n)
+    full, fulld = np.zeros_like(C), np.zeros_like(Cd)
+    for row, (i, j, k) in enumerate(modes):
+        full[i, j, :, k] = Cc[row]
+        fulld[i, j, :, k] = Cdc[row]
+    assert np.array_equal(full, C) and np.array_equal(fulld, Cd)
+    assert np.array_equal(tab[o + 2 * ns * n:], np.concatenate([xa, xb, xc, w.ravel(), s]))
