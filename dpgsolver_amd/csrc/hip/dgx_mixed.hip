@@ -646,8 +646,11 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
     const bool act = slot < t.count;
     const int64_t e = t.e0 + slot;
     constexpr int TAB = sf_tab_lds(DGX_ELEM_WEDGE, P);
-    const double* B1 = lds;
-    const double* D1 = lds + n2;
+    // the LINE factors are read at compile-time indices: scalar operands (as in the HEX path); the TRI
+    // tables at lane-dependent rows from the LDS copy
+    using cdp = const __attribute__((address_space(4))) double*;
+    const cdp B1 = (cdp)(a.ops + G.tp_off);
+    const cdp D1 = B1 + n2;
     const double* w1 = lds + 2 * n2;
     const double* tri = lds + 2 * n2 + n;                  // Bt, Gr, Gs [n2][nt], wt [n2]
     const double* Bt = tri;
@@ -725,11 +728,10 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
             }
         }
         __syncthreads();
-        // LINE^T: lines (d, v, qt), in place
+        // LINE^T: lines (d, v, qt), in place: B1^T for d = 0, 1, then D1^T for d = 2 (one operator per loop)
         if (act) {
-            for (int ln = l; ln < 15 * n2; ln += tpe) {
+            for (int ln = l; ln < 10 * n2; ln += tpe) {
                 const int dv = ln / n2, qt = ln - dv * n2;
-                const double* op = dv < 10 ? B1 : D1;
                 const int base = dv * n3 + qt;
                 double in[n];
 #pragma unroll
@@ -740,8 +742,24 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
                     double acc = 0.0;
 #pragma unroll
                     for (int q = 0; q < n; ++q)
-                        acc = fma(op[q * n + k], in[q], acc);
+                        acc = fma(B1[q * n + k], in[q], acc);
                     W[base + k * n2] = acc;         // [d][v][k][qt]
+                }
+            }
+            for (int ln = 10 * n2 + l; ln < 15 * n2; ln += tpe) {
+                const int dv = ln / n2, qt = ln - dv * n2;
+                const int base = dv * n3 + qt;
+                double in[n];
+#pragma unroll
+                for (int q = 0; q < n; ++q)
+                    in[q] = W[base + q * n2];
+#pragma unroll
+                for (int k = 0; k < n; ++k) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < n; ++q)
+                        acc = fma(D1[q * n + k], in[q], acc);
+                    W[base + k * n2] = acc;
                 }
             }
         }
@@ -959,8 +977,10 @@ __device__ __attribute__((always_inline)) inline void vol_tile_collapsed(const M
     const int64_t e = t.e0 + slot;
     constexpr int TAB = sf_tab_lds(TYPE, P);
     const double* tab = lds;      // the factor block, staged in LDS
-    const double* Al = tab;       // A, Ad [n][n]
-    const double* Adl = tab + n2;
+    // A, Ad are read at compile-time indices: scalar operands through the constant address space
+    using cdp = const __attribute__((address_space(4))) double*;
+    const cdp Al = (cdp)(a.ops + G.tp_off);
+    const cdp Adl = Al + n2;
     const double* Bt = tab + 2 * n2;
     const double* Bdt = Bt + K::size_b();
     const double* Ct = Bdt + K::size_b();
@@ -1100,11 +1120,10 @@ __device__ __attribute__((always_inline)) inline void vol_tile_collapsed(const M
             }
         }
         __syncthreads();
-        // transpose a: lines (comp, v, q, r), in place
+        // transpose a: lines (comp, v, q, r), in place: Ad^T for comp a, then A^T for b, c (one operator per loop)
         if (act) {
-            for (int ln = l; ln < 15 * n2; ln += tpe) {
+            for (int ln = l; ln < 5 * n2; ln += tpe) {
                 const int cv = ln / n2, rr = ln - cv * n2;
-                const double* op = cv < 5 ? Adl : Al;
                 const int base = cv * n3 + n * rr;
                 double in[n];
 #pragma unroll
@@ -1115,7 +1134,23 @@ __device__ __attribute__((always_inline)) inline void vol_tile_collapsed(const M
                     double acc = 0.0;
 #pragma unroll
                     for (int pa = 0; pa < n; ++pa)
-                        acc = fma(op[pa * n + i], in[pa], acc);
+                        acc = fma(Adl[pa * n + i], in[pa], acc);
+                    W[base + i] = acc;
+                }
+            }
+            for (int ln = 5 * n2 + l; ln < 15 * n2; ln += tpe) {
+                const int cv = ln / n2, rr = ln - cv * n2;
+                const int base = cv * n3 + n * rr;
+                double in[n];
+#pragma unroll
+                for (int pa = 0; pa < n; ++pa)
+                    in[pa] = W[base + pa];
+#pragma unroll
+                for (int i = 0; i < n; ++i) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int pa = 0; pa < n; ++pa)
+                        acc = fma(Al[pa * n + i], in[pa], acc);
                     W[base + i] = acc;
                 }
             }
