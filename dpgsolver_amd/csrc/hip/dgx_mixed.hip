@@ -185,8 +185,40 @@ __device__ inline double rk_mix(int stage, double dt, double s, double r, const 
 // Per-mode end of the volume phase: face lifts, M^-1, the residual or the RK update (the new state also
 // goes to U, the tile's LDS copy, for the trace phase).  s_old: the stage input value of each variable.
 __device__ __attribute__((always_inline)) inline void finish_mode(const MixArgs& a, int64_t e, int ns, int j, double r[5], const double s_old[5],
-                                   double* U, bool dense_lift = true)
+                                   double* U, bool dense_lift = true, bool wave_elem = false)
 {
+    if (dense_lift && wave_elem) {
+        // the wavefront serves one element: its face records and face fluxes are read as scalars (the
+        // same values on every lane; through vector memory they cost five loads per face node), the
+        // lift rows as vector loads over the modes
+        using cip = const __attribute__((address_space(4))) int32_t*;
+        using clp = const __attribute__((address_space(4))) int64_t*;
+        using cdp = const __attribute__((address_space(4))) double*;
+        const int eu = __builtin_amdgcn_readfirstlane((int)e);
+#pragma unroll 1
+        for (int k = 0; k < 6; ++k) {
+            const int ent = ((cip)a.ef)[6 * eu + k];
+            if (ent < 0)
+                break;
+            const int f = ent >> 1, side = ent & 1;
+            const int o = ((cip)a.face_op)[2 * f + side];
+            const int nq = ((cip)a.fop_nq)[o];
+            const double* bf = a.fop + ((clp)a.fop_off)[o];
+            const cdp nf = (cdp)a.fb + ((clp)a.fb_off)[f];
+            double s[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll 2
+            for (int q = 0; q < nq; ++q) {
+                const double b = bf[q * ns + j];
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    s[v] = fma(b, nf[5 * q + v], s[v]);
+            }
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                r[v] = side ? r[v] + s[v] : r[v] - s[v];
+        }
+        dense_lift = false;
+    }
 #pragma unroll 1
     for (int k = 0; k < 6 && dense_lift; ++k) {
         const int ent = a.ef[6 * e + k];
@@ -1228,7 +1260,7 @@ __device__ __attribute__((always_inline)) inline void vol_tile_collapsed(const M
                     r[v] = Um[v * nsr + j];
                     s_old[v] = ue[v * nsr + j];
                 }
-                finish_mode(a, e, nsr, j, r, s_old, Um);
+                finish_mode(a, e, nsr, j, r, s_old, Um, true, tpe >= 64);
             }
         }
         __syncthreads();
