@@ -46,6 +46,7 @@ struct GroupDev {
 };
 
 constexpr int kTpLds = 96;   // doubles at the start of a volume tile's LDS: the 1-D factors
+constexpr int kGroupedBpc = 4;   // workgroups of the volume kernels one CU holds at their register count
 
 // Modes of a (type, p) element of the sum-factorized paths.
 __host__ __device__ constexpr int sf_modes(int type, int p)
@@ -1384,10 +1385,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
 // The grouped form: one launch over the tiles of every group, one tile per workgroup, in the
 // persistent order (largest first); the hardware dispatcher balances the load, and without a work loop
 // the kernel keeps the register count of a single tile.
-__global__ __launch_bounds__(kBlock) void vol_grouped(MixArgs a)
+__global__ __launch_bounds__(kBlock) void vol_grouped(MixArgs a, int64_t t0)
 {
     extern __shared__ double lds[];
-    vol_dispatch(a, a.vtiles[a.vorder[blockIdx.x]], lds);
+    vol_dispatch(a, a.vtiles[a.vorder[t0 + blockIdx.x]], lds);
 }
 
 __global__ __launch_bounds__(kBlock) void vol_range(MixArgs a, int64_t t0)
@@ -1448,13 +1449,18 @@ struct dgx_mixed_ctx {
             *fopr_off = nullptr, *foprt_off = nullptr;
     int32_t *fopr_n = nullptr;
     int32_t *ef = nullptr, *elf = nullptr, *face_elem = nullptr, *face_op = nullptr, *face_wt = nullptr, *fop_nq = nullptr,
-            *fop_ns = nullptr, *face_order = nullptr, *vorder = nullptr;
+            *fop_ns = nullptr, *face_order = nullptr, *vorder = nullptr, *gorder = nullptr;
     GroupDev* d_groups = nullptr;
     VolTile* vtiles = nullptr;
     FaceTile* ftiles = nullptr;
     unsigned int* counter = nullptr;
     int64_t n_vtiles = 0, n_ftiles = 0;
     std::vector<int64_t> vtile_start;    // [n_group+1] per-group ranges of vtiles
+    struct GClass {
+        int64_t start, count;            // range of gorder
+        size_t lds;                      // the launch's LDS: the largest tile of the class
+    };
+    std::vector<GClass> gclass;          // grouped form: one launch per LDS class
     std::vector<int64_t> ftile_class;    // [k+1] ranges of face tiles of one lanes-per-face class
     size_t lds_max = 0;
     hipStream_t stream = nullptr;
@@ -1539,16 +1545,41 @@ dgx_status build_vtiles(dgx_mixed_ctx* c)
     for (size_t i = 0; i < order.size(); ++i)
         order[i] = (int32_t)i;
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+    // grouped form: one launch per LDS class, the workgroups a CU can hold at the class's largest tile
+    // (at most kGroupedBpc, the register limit), so that one large-tile group does not set the LDS, and
+    // with it the occupancy, of every other group's workgroups
+    auto bpc_of = [](size_t lds) {
+        return std::max(1, std::min(kGroupedBpc, (int)((160u << 10) / std::max<size_t>(lds, 1))));
+    };
+    std::vector<int32_t> gorder(order);
+    std::stable_sort(gorder.begin(), gorder.end(), [&](int32_t x, int32_t y) {
+        return bpc_of(tile_lds(c->groups[vt[x].group], c->groups[vt[x].group].tile)) <
+               bpc_of(tile_lds(c->groups[vt[y].group], c->groups[vt[y].group].tile));
+    });
+    c->gclass.clear();
+    for (size_t i = 0; i < gorder.size(); ++i) {
+        const GroupDev& G = c->groups[vt[gorder[i]].group];
+        const size_t l = tile_lds(G, G.tile);
+        if (c->gclass.empty() || bpc_of(c->gclass.back().lds) != bpc_of(l))
+            c->gclass.push_back({ (int64_t)i, 0, l });
+        c->gclass.back().count += 1;
+        c->gclass.back().lds = std::max(c->gclass.back().lds, l);
+    }
     if (c->vtiles)
         (void)hipFree(c->vtiles);
     if (c->vorder)
         (void)hipFree(c->vorder);
+    if (c->gorder)
+        (void)hipFree(c->gorder);
     c->vtiles = nullptr;
     c->vorder = nullptr;
+    c->gorder = nullptr;
     hipError_t e = hipSuccess;
     c->vtiles = dev_copy(vt, &e);
     MIX_TRY(c, e);
     c->vorder = dev_copy(order, &e);
+    MIX_TRY(c, e);
+    c->gorder = dev_copy(gorder, &e);
     MIX_TRY(c, e);
     c->n_vtiles = (int64_t)vt.size();
     std::vector<GroupDev> gd = c->groups;
@@ -1646,9 +1677,17 @@ hipError_t launch_vol(dgx_mixed_ctx* c, const MixArgs& a, int mode, int only_gro
         return hipGetLastError();
     }
     if (mode == DGX_MIXED_LAUNCH_GROUPED && only_group < 0) {
-        if (c->n_vtiles > 0)
-            vol_grouped<<<(unsigned)c->n_vtiles, kBlock, c->lds_max, c->stream>>>(a);
-        return hipGetLastError();
+        MixArgs ag = a;
+        ag.vorder = c->gorder;
+        for (const auto& k : c->gclass) {
+            if (k.count <= 0)
+                continue;
+            vol_grouped<<<(unsigned)k.count, kBlock, k.lds, c->stream>>>(ag, k.start);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
     }
     for (int g = 0; g < c->n_group; ++g) {
         if (only_group >= 0 && g != only_group)
@@ -1742,8 +1781,11 @@ void free_all(dgx_mixed_ctx* c)
         (void)hipFree(c->vtiles);
     if (c->vorder)
         (void)hipFree(c->vorder);
+    if (c->gorder)
+        (void)hipFree(c->gorder);
     c->vtiles = nullptr;
     c->vorder = nullptr;
+    c->gorder = nullptr;
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
