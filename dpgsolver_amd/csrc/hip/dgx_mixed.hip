@@ -207,7 +207,7 @@ __device__ __attribute__((always_inline)) inline void finish_mode(const MixArgs&
             const double* bf = a.fop + ((clp)a.fop_off)[o];
             const cdp nf = (cdp)a.fb + ((clp)a.fb_off)[f];
             double s[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-#pragma unroll 2
+#pragma unroll 6
             for (int q = 0; q < nq; ++q) {
                 const double b = bf[q * ns + j];
 #pragma unroll
@@ -231,7 +231,7 @@ __device__ __attribute__((always_inline)) inline void finish_mode(const MixArgs&
         const double* bf = a.fop + a.fop_off[o];
         const double* nf = a.fb + a.fb_off[f];
         double s[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-#pragma unroll 2
+#pragma unroll 6
         for (int q = 0; q < nq; ++q) {
             const double b = bf[q * ns + j];
 #pragma unroll
@@ -283,8 +283,8 @@ __device__ __attribute__((always_inline)) inline void write_traces(const MixArgs
         const int o = a.face_op[2 * f + side];
         const double* bt = a.fop + a.fopt_off[o];
         double w[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-        // not unrolled: ns is a compile-time constant on the sum-factorized paths
-#pragma unroll 2
+        // partly unrolled: ns is a compile-time constant on the sum-factorized paths
+#pragma unroll 6
         for (int j = 0; j < ns; ++j) {
             const double b = bt[j * nq + q];
 #pragma unroll
@@ -563,6 +563,9 @@ __device__ __attribute__((always_inline)) inline void vol_tile_hex(const MixArgs
                 const double* bf = a.fop + a.fopr_off[o];
                 const double* nf = a.fb + a.fb_off[f];
                 double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+                // unrolled so that several face nodes' loads are in flight (one at a time, the loop waits
+                // a memory latency per node)
+#pragma unroll 6
                 for (int q = 0; q < nq; ++q) {
                     const double b = bf[q * n2 + ab];
 #pragma unroll
@@ -645,7 +648,7 @@ __device__ __attribute__((always_inline)) inline void vol_tile_hex(const MixArgs
                 const int o = a.face_op[2 * f + side];
                 const double* bt = a.fop + a.foprt_off[o];
                 double w[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-#pragma unroll 4
+#pragma unroll 6
                 for (int ab = 0; ab < n2; ++ab) {
                     const double b = bt[ab * nq + q];
 #pragma unroll
@@ -829,6 +832,7 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
                 const double* bf = a.fop + a.fopr_off[o];
                 const double* nf = a.fb + a.fb_off[f];
                 double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll 6
                 for (int q = 0; q < nq; ++q) {
                     const double b = bf[q * wdt + x];
 #pragma unroll
@@ -925,6 +929,7 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
                 const int wdt = a.fopr_n[o];
                 const double* bt = a.fop + a.foprt_off[o];
                 double w[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll 6
                 for (int x = 0; x < wdt; ++x) {
                     const double b = bt[x * nq + q];
 #pragma unroll
