@@ -120,6 +120,8 @@ struct MixArgs {
     const double* face_jac;
     const int64_t* fb_off;
     double* fb;            // face flux buffer [sum Nq][5]
+    const int64_t* fl_off; // [2F]: the side's face-modal lift coefficients in fl, -1 for sides lifted densely
+    double* fl;            // face-modal lift coefficients of HEX / WEDGE sides, [5][width] each
     double* tr;            // trace buffer: face f at 2 fb_off[f]: [side][Nq][5]
     const int32_t* fop_nq;
     const int32_t* fop_ns;
@@ -147,31 +149,59 @@ struct MixArgs {
 template <int FLUX>
 __device__ __attribute__((always_inline)) inline void face_tile(const MixArgs& a, const FaceTile& t)
 {
-    const int lg = t.lg_tpf;
-    const int slot = threadIdx.x >> lg, l = threadIdx.x & ((1 << lg) - 1);
-    if (slot >= t.count)
-        return;
-    const int f = a.face_order[t.i0 + slot];
-    const int oL = a.face_op[2 * f], oR = a.face_op[2 * f + 1];
-    const int nq = a.fop_nq[oL];
-    if (l >= nq)
-        return;
-    (void)oR;
-    double wl[5], wr[5];
-    const double* tf = a.tr + 2 * a.fb_off[f] + 5 * l;
+    __shared__ double s_nf[kBlock * 5];   // the tile's scaled fluxes, [lane][v]
+    const int lg = t.lg_tpf, tpf = 1 << lg;
+    const int slot = threadIdx.x >> lg, l = threadIdx.x & (tpf - 1);
+    const bool face = slot < t.count;
+    const int f = face ? a.face_order[t.i0 + slot] : 0;
+    const int nq = face ? a.fop_nq[a.face_op[2 * f]] : 0;
+    if (face && l < nq) {
+        double wl[5], wr[5];
+        const double* tf = a.tr + 2 * a.fb_off[f] + 5 * l;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        wl[v] = tf[v];
-        wr[v] = tf[5 * nq + v];
+        for (int v = 0; v < 5; ++v) {
+            wl[v] = tf[v];
+            wr[v] = tf[5 * nq + v];
+        }
+        const double n[3] = { a.face_normal[3 * f], a.face_normal[3 * f + 1], a.face_normal[3 * f + 2] };
+        double nf[5];
+        dgx::numflux<3, FLUX>(wl, wr, n, nf);
+        const double sc = a.fw[a.fw_off[a.face_wt[f]] + l] * a.face_jac[f];
+        double* o = a.fb + a.fb_off[f] + 5 * l;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            o[v] = nf[v] * sc;
+            s_nf[5 * threadIdx.x + v] = nf[v] * sc;
+        }
     }
-    const double n[3] = { a.face_normal[3 * f], a.face_normal[3 * f + 1], a.face_normal[3 * f + 2] };
-    double nf[5];
-    dgx::numflux<3, FLUX>(wl, wr, n, nf);
-    const double s = a.fw[a.fw_off[a.face_wt[f]] + l] * a.face_jac[f];
-    double* o = a.fb + a.fb_off[f] + 5 * l;
+    __syncthreads();
+    // the lift of each HEX / WEDGE side in face-modal form, -+sum_q BF[q][x] nnf[q][v] (the left side takes
+    // -nnf): computed here, where the face's fluxes sit in LDS, the volume kernel reads 5 x width values
+    // per face instead of a vector load per face node and row
+    if (face) {
+        const int64_t offL = a.fl_off[2 * f], offR = a.fl_off[2 * f + 1];
+        const int oL = a.face_op[2 * f], oR = a.face_op[2 * f + 1];
+        const int wL = offL >= 0 ? a.fopr_n[oL] : 0, wR = offR >= 0 ? a.fopr_n[oR] : 0;
+        const double* nfs = s_nf + 5 * (slot << lg);
+        for (int it = l; it < wL + wR; it += tpf) {
+            const int side = it >= wL ? 1 : 0;
+            const int x = it - side * wL, wdt = side ? wR : wL;
+            const double* bf = a.fop + a.fopr_off[side ? oR : oL];
+            double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll 6
+            for (int q = 0; q < nq; ++q) {
+                const double b = bf[q * wdt + x];
 #pragma unroll
-    for (int v = 0; v < 5; ++v)
-        o[v] = nf[v] * s;
+                for (int v = 0; v < 5; ++v)
+                    acc[v] = fma(b, nfs[5 * q + v], acc[v]);
+            }
+            double* out = a.fl + (side ? offR : offL) + x;
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                out[v * wdt] = side ? acc[v] : -acc[v];
+        }
+    }
+    __syncthreads();   // s_nf is reused by the next tile of a persistent workgroup
 }
 
 __device__ inline double rk_mix(int stage, double dt, double s, double r, const double* reg)
@@ -552,29 +582,15 @@ __device__ __attribute__((always_inline)) inline void vol_tile_hex(const MixArgs
             }
         }
         __syncthreads();
-        // face lifts in face-modal form: S_k[v][ab] = sum_q BF_k[q][ab] nnf_k[q][v] (lanes over (k, ab))
+        // face lifts in face-modal form, S_k[v][ab] = -+sum_q BF_k[q][ab] nnf_k[q][v], as the face kernel left
+        // them (lanes over (k, ab))
         if (act) {
             for (int it = l; it < 6 * n2; it += tpe) {
                 const int k = it / n2, ab = it - k * n2;
-                const int ent = a.ef[6 * e + k];
-                const int f = ent >> 1, side = ent & 1;
-                const int o = a.face_op[2 * f + side];
-                const int nq = a.fop_nq[o];
-                const double* bf = a.fop + a.fopr_off[o];
-                const double* nf = a.fb + a.fb_off[f];
-                double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-                // unrolled so that several face nodes' loads are in flight (one at a time, the loop waits
-                // a memory latency per node)
-#pragma unroll 6
-                for (int q = 0; q < nq; ++q) {
-                    const double b = bf[q * n2 + ab];
-#pragma unroll
-                    for (int v = 0; v < 5; ++v)
-                        acc[v] = fma(b, nf[5 * q + v], acc[v]);
-                }
+                const double* fl = a.fl + a.fl_off[a.ef[6 * e + k]] + ab;
 #pragma unroll
                 for (int v = 0; v < 5; ++v)
-                    S[(k * 5 + v) * n2 + ab] = side ? acc[v] : -acc[v];
+                    S[(k * 5 + v) * n2 + ab] = fl[v * n2];
             }
         }
         __syncthreads();
@@ -818,30 +834,19 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
             }
         }
         __syncthreads();
-        // face lifts in face-modal form (the nodal work area is free): S_k[v][x] = sum_q BF_k[q][x] nnf[q][v]
+        // face lifts in face-modal form (the nodal work area is free), S_k[v][x] = -+sum_q BF_k[q][x] nnf[q][v],
+        // as the face kernel left them
         if (act) {
             for (int it = l; it < 5 * n2; it += tpe) {
                 const int k = it / n2, x = it - k * n2;
                 const int ent = a.ef[6 * e + k];
-                const int f = ent >> 1, side = ent & 1;
-                const int o = a.face_op[2 * f + side];
-                const int wdt = a.fopr_n[o];
+                const int wdt = a.fopr_n[a.face_op[ent]];
                 if (x >= wdt)
                     continue;
-                const int nq = a.fop_nq[o];
-                const double* bf = a.fop + a.fopr_off[o];
-                const double* nf = a.fb + a.fb_off[f];
-                double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-#pragma unroll 6
-                for (int q = 0; q < nq; ++q) {
-                    const double b = bf[q * wdt + x];
-#pragma unroll
-                    for (int v = 0; v < 5; ++v)
-                        acc[v] = fma(b, nf[5 * q + v], acc[v]);
-                }
+                const double* fl = a.fl + a.fl_off[ent] + x;
 #pragma unroll
                 for (int v = 0; v < 5; ++v)
-                    S[(k * 5 + v) * n2 + x] = side ? acc[v] : -acc[v];
+                    S[(k * 5 + v) * n2 + x] = fl[v * wdt];
             }
         }
         __syncthreads();
@@ -1450,6 +1455,9 @@ struct dgx_mixed_ctx {
     double *u = nullptr, *up = nullptr, *rhs = nullptr, *fb = nullptr, *tr = nullptr;
     double *ops = nullptr, *jdet = nullptr, *met = nullptr, *fnorm = nullptr, *fjac = nullptr, *fop = nullptr,
            *fw = nullptr;
+    int64_t fl_len = 0;
+    int64_t *fl_off = nullptr;
+    double* fl = nullptr;
     int64_t *dof = nullptr, *fb_off = nullptr, *fop_off = nullptr, *fopt_off = nullptr, *fw_off = nullptr,
             *fopr_off = nullptr, *foprt_off = nullptr;
     int32_t *fopr_n = nullptr;
@@ -1615,6 +1623,8 @@ MixArgs make_args(dgx_mixed_ctx* c, const double* u, const double* reg, double* 
     a.face_jac = c->fjac;
     a.fb_off = c->fb_off;
     a.fb = c->fb;
+    a.fl_off = c->fl_off;
+    a.fl = c->fl;
     a.tr = c->tr;
     a.fop_nq = c->fop_nq;
     a.fop_ns = c->fop_ns;
@@ -1931,8 +1941,8 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
         if (c->groups[g].tp_off >= 0 && (c->groups[g].type == DGX_ELEM_HEX || c->groups[g].type == DGX_ELEM_WEDGE))
             for (int64_t e = c->groups[g].start; e < c->groups[g].start + c->groups[g].count; ++e)
                 hex_tp[e] = (char)(1 + c->groups[g].type);
-    std::vector<int64_t> fb_off(c->F);
-    int64_t fb = 0;
+    std::vector<int64_t> fb_off(c->F), fl_off(2 * c->F, -1);
+    int64_t fb = 0, fl = 0;
     c->face_lg_of.resize(c->F);
     for (int64_t f = 0; f < c->F; ++f) {
         const int32_t eL = m->face_elem[2 * f], eR = m->face_elem[2 * f + 1];
@@ -1958,6 +1968,9 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
                           "face-modal row width differs from the side's face");
                 for (int k = 0; k < nfe[e]; ++k)
                     MIX_CHECK(c, elf[6 * e + k] != lf, "two faces of one element on the same local face");
+                // the face kernel projects the side's flux onto its face-modal rows: [5][width]
+                fl_off[2 * f + s] = fl;
+                fl += 5LL * fn2[o];
             }
             elf[6 * e + nfe[e]] = lf;
             ef[6 * e + nfe[e]++] = (int32_t)(2 * f + s);
@@ -1969,6 +1982,7 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     for (int64_t e = 0; e < c->E; ++e)
         MIX_CHECK(c, nfe[e] == kFacesOf[type_of[e]], "element with a face missing (the mesh must be closed)");
     c->fb_len = fb;
+    c->fl_len = fl;
     // face tiles: faces ordered by lanes-per-face class
     std::vector<int32_t> order(c->F);
     for (int64_t f = 0; f < c->F; ++f)
@@ -2015,6 +2029,7 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     MIX_RUN(upload(c, &c->fnorm, std::vector<double>(m->face_normal, m->face_normal + 3 * c->F)));
     MIX_RUN(upload(c, &c->fjac, std::vector<double>(m->face_jac, m->face_jac + c->F)));
     MIX_RUN(upload(c, &c->fb_off, fb_off));
+    MIX_RUN(upload(c, &c->fl_off, fl_off));
     MIX_RUN(upload(c, &c->fop_nq, fnq));
     MIX_RUN(upload(c, &c->fop_ns, fns));
     MIX_RUN(upload(c, &c->fop_off, foff));
@@ -2033,6 +2048,7 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     MIX_RUN(alloc(c, &c->up, c->n_dof));
     MIX_RUN(alloc(c, &c->rhs, c->n_dof));
     MIX_RUN(alloc(c, &c->fb, c->fb_len));
+    MIX_RUN(alloc(c, &c->fl, std::max<int64_t>(c->fl_len, 1)));
     MIX_RUN(alloc(c, &c->tr, 2 * c->fb_len));
     MIX_RUN(alloc(c, &c->counter, 2));
     MIX_TRY(c, hipMemset(c->u, 0, c->n_dof * sizeof(double)));
