@@ -724,16 +724,22 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
     }
     __syncthreads();
     if (a.stage != -2) {
-        // TRI: lines (v, k)
+        // TRI: lines (k, all 5 variables): one operator read serves the 5 variables
         if (act) {
-            for (int it = l; it < 5 * n * n2; it += tpe) {
-                const int vk = it / n2, qt = it - vk * n2;
-                const double* in = Um + vk * nt;
-                double acc = 0.0;
-#pragma unroll 4
-                for (int m = 0; m < nt; ++m)
-                    acc = fma(Bt[qt * nt + m], in[m], acc);
-                W[vk * n2 + qt] = acc;              // [v][k][qt]
+            for (int it = l; it < n * n2; it += tpe) {
+                const int k = it / n2, qt = it - k * n2;
+                const double* in = Um + k * nt;
+                double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll 3
+                for (int m = 0; m < nt; ++m) {
+                    const double b = Bt[qt * nt + m];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        acc[v] = fma(b, in[v * ns + m], acc[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    W[(v * n + k) * n2 + qt] = acc[v];   // [v][k][qt]
             }
         }
         __syncthreads();
@@ -816,21 +822,27 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
             }
         }
         __syncthreads();
-        // TRI^T: outputs (v, k, m) -> Um
+        // TRI^T: outputs (k, m) of all 5 variables -> Um (the three table reads serve the 5 variables)
         if (act) {
-            for (int it = l; it < 5 * ns; it += tpe) {
-                const int vk = it / nt, mm = it - vk * nt;
-                const double* y0 = W + vk * n2;                 // d = 0: [v][k] block
-                const double* y1 = W + 5 * n3 + vk * n2;
-                const double* y2 = W + 10 * n3 + vk * n2;
-                double acc = 0.0;
-#pragma unroll 4
+            for (int it = l; it < ns; it += tpe) {
+                const int k = it / nt, mm = it - k * nt;
+                const double* y0 = W + k * n2;                  // d = 0: [v][k] blocks, v stride n3
+                const double* y1 = W + 5 * n3 + k * n2;
+                const double* y2 = W + 10 * n3 + k * n2;
+                double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll 2
                 for (int qt = 0; qt < n2; ++qt) {
-                    acc = fma(Gr[qt * nt + mm], y0[qt], acc);
-                    acc = fma(Gs[qt * nt + mm], y1[qt], acc);
-                    acc = fma(Bt[qt * nt + mm], y2[qt], acc);
+                    const double gr = Gr[qt * nt + mm], gs = Gs[qt * nt + mm], bt = Bt[qt * nt + mm];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        acc[v] = fma(gr, y0[v * n3 + qt], acc[v]);
+                        acc[v] = fma(gs, y1[v * n3 + qt], acc[v]);
+                        acc[v] = fma(bt, y2[v * n3 + qt], acc[v]);
+                    }
                 }
-                Um[it] = acc;
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    Um[v * ns + it] = acc[v];
             }
         }
         __syncthreads();
