@@ -61,7 +61,8 @@ typedef struct dgx_mixed_mesh {
      * r, s derivatives, Nt = n(n+1)/2), wt [n^2], E [3][n][Nt] (the TRI modes on the edge under quad face e
      * in the edge's Legendre basis), tm, tp [n]; B = B1 (x) Bt (TRI fastest).
      * TET, PYR (collapsed coordinates a, b, c): A, Ad [n][n]; B, Bd (TET [n][n][n] as [i][q][j], PYR
-     * [n][n]); C, Cd (TET [n][n][n][n] as [i][j][r][k], PYR [n][n][n] as [max(i,j)][r][k]); the nodes
+     * [n][n]); C, Cd (TET [Ns][n]: row = mode (i, j, k) in the mode order, i slowest, k fastest, then r -
+     * only the (i, j, k) entries of its modes; PYR [n][n][n] as [max(i,j)][r][k]); the nodes
      * xa, xb, xc [n]; the node weights [n^3]; the mode scales s [Ns]: phi = s A B C. */
     const int64_t* vol_tp_off;
     const double* jdet;          /* [n_elem] det of the affine map */
