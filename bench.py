@@ -9,9 +9,11 @@ Prints one JSON line:
   value    = DOF-stages/s (DOF = E * (p+1)^3 * 5), whole job
   roofline = the stage kernel against HBM: algorithmic bytes per stage launch / measured stage time,
              against the 8 TB/s peak (frac) and against a measured 2 GiB device copy (frac_of_measured_copy)
-  cpu_baseline = the oracle (dense CPU restatement of the reference algorithm) on a bounded sample of the
-             same workload (a 16^3 mesh of the same construction), in DOF-stages/s, on all host threads
-             (value, cores) and on one thread (single_thread; the reference is single-threaded)
+  cpu_baseline = the oracle (CPU restatement of the reference algorithm) on a bounded sample of the
+             same workload (a 16^3 mesh of the same construction), in DOF-stages/s: sum-factorized element
+             operators on all host threads (value, cores) and on one thread; dense per-element operators
+             (the reference's form) on all host threads (dense) and on one thread (single_thread; the
+             reference is single-threaded)
 """
 import argparse
 import json
@@ -74,8 +76,11 @@ def cpu_model():
 
 
 def cpu_baseline(args, d, p, num_flux):
-    """Time the oracle's rk stages on a 16^3 (3-D) / 32^2 (2-D) sample: one thread (the reference is
-    single-threaded) and all host threads (OpenMP over volumes, faces in two phases)."""
+    """Time the oracle's rk stages on a 16^3 (3-D) / 32^2 (2-D) sample in its two operator forms: the
+    dense per-element operators (the reference's form; one thread, as the reference runs, and all host
+    threads) and the sum-factorized form (Kronecker factors applied one direction at a time, diagonal
+    M^-1; all host threads and one thread).  `value` is the fastest of them, sum-factorized on all host
+    threads (OpenMP over volumes, faces in two phases)."""
     import numpy as np
     import dpgsolver_amd as dg
     from oracle.oracle import Oracle
@@ -91,8 +96,9 @@ def cpu_baseline(args, d, p, num_flux):
     dt = dg.dt_cfl(mesh, p, u0)
     dof = mesh.n_elem * (p + 1) ** d * (d + 2)
 
-    def rate(threads, seconds):
+    def rate(threads, seconds, sf):
         orc.set_threads(threads)
+        orc.set_sum_factorized(sf)
         u, up = u0.copy(), np.zeros_like(u0)
         stages = 0
         t0 = time.perf_counter()
@@ -103,14 +109,24 @@ def cpu_baseline(args, d, p, num_flux):
             if el > seconds and stages % 3 == 0:
                 return dof * stages / el, stages, el
 
-    r1, s1, e1 = rate(1, args.cpu_seconds)
     nt = host_threads()
-    rn, sn, en = rate(nt, args.cpu_seconds / 2) if nt > 1 else (r1, s1, e1)
+    part = args.cpu_seconds / 4
+    d1 = rate(1, part, False)
+    dn = rate(nt, part, False) if nt > 1 else d1
+    f1 = rate(1, part, True)
+    fn = rate(nt, part, True) if nt > 1 else f1
+    orc.set_sum_factorized(False)
     host = f"{cpu_model()}, {os.cpu_count()} logical CPUs, {len(os.sched_getaffinity(0))} available"
-    return {"value": rn, "unit": "DOF-stages/s", "cores": nt, "kind": "port",
-            "sample": f"{sample}; {sn} stages in {en:.1f} s on {nt} threads (oracle, dense per-element "
-                      f"operators; {host})",
-            "single_thread": {"value": r1, "cores": 1, "sample": f"{s1} stages in {e1:.1f} s"}}
+
+    def sub(r, threads, form):
+        return {"value": r[0], "cores": threads, "sample": f"{r[1]} stages in {r[2]:.1f} s ({form})"}
+
+    return {"value": fn[0], "unit": "DOF-stages/s", "cores": nt, "kind": "port",
+            "sample": f"{sample}; {fn[1]} stages in {fn[2]:.1f} s on {nt} threads (oracle, sum-factorized "
+                      f"element operators; {host})",
+            "sum_factorized_single_thread": sub(f1, 1, "sum-factorized"),
+            "dense": sub(dn, nt, "dense per-element operators"),
+            "single_thread": sub(d1, 1, "dense per-element operators, the reference's form")}
 
 
 PROTOCOL_STEPS = 50   # SSP-RK3 steps of the state comparisons of the 64^3 workload
@@ -681,7 +697,7 @@ def main():
     ap.add_argument("--oracle-check", action="store_true",
                     help="also compare the state after the protocol's steps with the oracle (minutes of host time)")
     ap.add_argument("--cpu-sample-n", type=int, default=16)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--traffic-file", default=os.path.join(HERE, "profiles", "stage_hex_p3_64_traffic.md"))
     args = ap.parse_args()
 

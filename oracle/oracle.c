@@ -49,7 +49,39 @@ struct orc_ctx {
     int8_t* face_ord;           /* [F] neigh_info[1].ind_ord (NULL: every face aligned) */
     int64_t* ef_face;           /* [E][2d] face of each local face, built on first threaded use */
     int8_t* ef_side;            /* [E][2d] 0: the volume is the face's left volume, 1: its right */
+    /* sum-factorized application (orc_set_sum_factorized): every element operator above is a Kronecker
+     * product of these 1-D factors, applied by one mode product per direction instead of densely; M^-1 is
+     * the diagonal 1/(w J) of the GL collocation (cv0 = I) */
+    int sf;
+    double f_cv0[256], f_cv1[256], f_tw0[256], f_tw1[256], f_fm[16], f_fp[16], f_B[256], f_Binv[256];
+    double* minv_diag;          /* [E][Ns], with the geometry */
 };
+
+/* operators of the sum-factorized application */
+enum { OP_CV0_VC, OP_TW1_VC, OP_CV1_VC, OP_TW0_VC, OP_CV0_FC, OP_TW0_FC };
+
+/* 1-D factors F[k] (row-major m[k] x n[k]) of operator op (direction or local face j): the factors
+ * orc_create builds the dense operator from */
+static void kron_of(const orc_ctx* c, int op, int j, const double** F, int* m, int* n)
+{
+    const int nn = c->n1;
+    for (int k = 0; k < 3; ++k) {
+        m[k] = nn;
+        n[k] = nn;
+        F[k] = (op == OP_TW1_VC || op == OP_TW0_VC || op == OP_TW0_FC) ? c->f_tw0 : c->f_cv0;
+    }
+    if (op == OP_TW1_VC)
+        F[j] = c->f_tw1;
+    else if (op == OP_CV1_VC)
+        F[j] = c->f_cv1;
+    else if (op == OP_CV0_FC) {
+        F[j / 2] = (j & 1) ? c->f_fp : c->f_fm;
+        m[j / 2] = 1;
+    } else if (op == OP_TW0_FC) {
+        F[j / 2] = (j & 1) ? c->f_fp : c->f_fm;   /* the trace row as a column: tw0_vt_fc = cv0_vs_fc^T diag(w_fc) */
+        n[j / 2] = 1;
+    }
+}
 
 /* ------------------------------------------------------------------------------------------------ */
 /* small dense linear algebra                                                                        */
@@ -591,6 +623,14 @@ orc_ctx* orc_create(int d, int p, int p_g, int64_t n_elem, int64_t n_face, const
         }
     invert(n, B, Binv);
 
+    memcpy(c->f_cv0, cv0, sizeof(double) * n * n);
+    memcpy(c->f_cv1, cv1, sizeof(double) * n * n);
+    memcpy(c->f_tw0, tw0, sizeof(double) * n * n);
+    memcpy(c->f_tw1, tw1, sizeof(double) * n * n);
+    memcpy(c->f_fm, fm, sizeof(double) * n);
+    memcpy(c->f_fp, fp, sizeof(double) * n);
+    memcpy(c->f_B, B, sizeof(double) * n * n);
+    memcpy(c->f_Binv, Binv, sizeof(double) * n * n);
     const int ms[3] = { n, n, n };
     c->cv0_vs_vc = (double*)malloc(sizeof(double) * Nc * Ns);
     {
@@ -672,6 +712,7 @@ void orc_destroy(orc_ctx* c)
     if (!c)
         return;
     free(c->face_elem); free(c->face_lf); free(c->face_ord);
+    free(c->minv_diag);
     free(c->cv0_vs_vc);
     for (int j = 0; j < 3; ++j) {
         free(c->tw1_vt_vc[j]);
@@ -704,6 +745,15 @@ void orc_geometry(const orc_ctx* c, double* metrics_vc, double* jdet_vc, double*
 static void build_minv(orc_ctx* c)
 {
     const int Ns = c->Ns, Nc = c->Nc;
+    if (c->sf) {
+        if (!c->minv_diag)
+            c->minv_diag = (double*)malloc(sizeof(double) * (size_t)c->n_elem * Ns);
+        for (int64_t e = 0; e < c->n_elem; ++e)
+            for (int i = 0; i < Ns; ++i)
+                c->minv_diag[e * Ns + i] = 1.0 / (c->w_vc[i] * c->jdet_vc[e * Nc + i]);
+        c->minv_ready = 1;
+        return;
+    }
     double* m = (double*)malloc(sizeof(double) * Ns * Ns);
     for (int64_t e = 0; e < c->n_elem; ++e) {
         for (int i = 0; i < Ns; ++i)
@@ -838,8 +888,14 @@ static int positivity_elem(const orc_ctx* c, double* ue, double* sb)
 {
     const int d = c->d, nv = d + 2, Ns = c->Ns;
     int ok = 1;
-    for (int v = 0; v < nv; ++v)
-        dense_apply(Ns, Ns, c->ccSB0, ue + v * Ns, sb + v * Ns);
+    if (c->sf) {
+        const double* F[3] = { c->f_Binv, c->f_Binv, c->f_Binv };
+        const int m3[3] = { c->n1, c->n1, c->n1 };
+        kron_apply_d(d, m3, m3, F, nv, 1.0, ue, Ns, sb, Ns, 0);
+    } else {
+        for (int v = 0; v < nv; ++v)
+            dense_apply(Ns, Ns, c->ccSB0, ue + v * Ns, sb + v * Ns);
+    }
     cons_to_prim(d, Ns, sb);
     for (int vr = 0; vr < nv; vr += nv - 1) {
         const double* vd = sb + vr * Ns;
@@ -874,8 +930,14 @@ static int positivity_elem(const orc_ctx* c, double* ue, double* sb)
     if (!ok)
         return 0;
     prim_to_cons(d, Ns, sb);
-    for (int v = 0; v < nv; ++v)
-        dense_apply(Ns, Ns, c->ccBS0, sb + v * Ns, ue + v * Ns);
+    if (c->sf) {
+        const double* F[3] = { c->f_B, c->f_B, c->f_B };
+        const int m3[3] = { c->n1, c->n1, c->n1 };
+        kron_apply_d(d, m3, m3, F, nv, 1.0, sb, Ns, ue, Ns, 0);
+    } else {
+        for (int v = 0; v < nv; ++v)
+            dense_apply(Ns, Ns, c->ccBS0, sb + v * Ns, ue + v * Ns);
+    }
     return 1;
 }
 
@@ -996,6 +1058,20 @@ int orc_set_time_stepper(orc_ctx* c, int time_stepper)
 int orc_n_stages(const orc_ctx* c)
 {
     return c->time_stepper == 1 ? 1 : (c->time_stepper == 2 ? 5 : 3);
+}
+
+int orc_set_sum_factorized(orc_ctx* c, int on)
+{
+    if (on) {
+        /* the diagonal M^-1 needs the GL collocation: the Lagrange basis at its own nodes (cv0 = I) */
+        for (int i = 0; i < c->n1; ++i)
+            for (int k = 0; k < c->n1; ++k)
+                if (fabs(c->f_cv0[i * c->n1 + k] - (i == k ? 1.0 : 0.0)) > 1e-14)
+                    return -1;
+    }
+    c->sf = on ? 1 : 0;
+    c->minv_ready = 0;
+    return 0;
 }
 
 void orc_set_threads(orc_ctx* c, int nthreads)

@@ -64,6 +64,12 @@ void orc_set_penalty(orc_ctx* c, double eta);
  * are accumulated per volume in local-face order instead of the reference's face order (round-off
  * differences only); used to time the baseline on all host cores. */
 void orc_set_threads(orc_ctx* c, int nthreads);
+/* Sum-factorized application of the element operators (on != 0): every operator the residual applies is
+ * a Kronecker product of 1-D factors and is applied by one mode product per direction, M^-1 as the
+ * diagonal 1/(w J) of the Gauss-Legendre collocation, the positivity's Bezier maps likewise; the same
+ * operators as the dense path up to summation order.  A CPU baseline stronger than the reference's dense
+ * per-element operators.  Returns -1 (and leaves the dense path) when the basis is not collocated. */
+int orc_set_sum_factorized(orc_ctx* c, int on);
 
 /* Weak gradients of the Navier-Stokes path at state u, each [E][d][nv][Ns]: grad_v (volume term) and
  * grad (volume term plus the liftings of all faces of the element). */

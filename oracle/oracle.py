@@ -31,6 +31,7 @@ _SIGS = {
     "orc_gradient": (None, [ctypes.c_void_p, _dp, _dp, _dp]),
     "orc_set_penalty": (None, [ctypes.c_void_p, ctypes.c_double]),
     "orc_set_threads": (None, [ctypes.c_void_p, ctypes.c_int]),
+    "orc_set_sum_factorized": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "orc_operator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _dp]),
     "orc_residual": (ctypes.c_double, [ctypes.c_void_p, _dp, _dp, ctypes.c_int]),
     "orc_jvp_cs": (None, [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp, ctypes.c_int]),
@@ -218,6 +219,12 @@ class Oracle:
         """OpenMP threads of residual / rk_stage (1 = the reference's sequential order; more threads
         change the face-term summation order, i.e. round-off only)."""
         load().orc_set_threads(self._ptr, int(n))
+
+    def set_sum_factorized(self, on=True):
+        """Sum-factorized application of the element operators (the same operators up to summation order;
+        a stronger CPU baseline than the dense per-element operators)."""
+        if load().orc_set_sum_factorized(self._ptr, int(bool(on))) != 0:
+            raise ValueError("sum-factorized application needs a collocated (GL) basis")
 
     def gradient(self, u):
         """(grad_v, grad), each [E][d][nv][Ns]: weak volume gradient and the lifted gradient."""
