@@ -166,6 +166,8 @@ SIGNATURES = {
                                             c_int32_p]),
     "dgx_mixed_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p]),
     "dgx_mixed_persistent_waves": (ctypes.c_int, [ctypes.c_void_p]),
+    "dgx_mixed_set_face_maps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dgx_mixed_tensor_face_ops": (ctypes.c_int32, [ctypes.c_void_p]),
     "dgx_mixed_time_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, c_double_p]),
 }
 

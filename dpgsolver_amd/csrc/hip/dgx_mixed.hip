@@ -130,6 +130,9 @@ struct MixArgs {
     const int64_t* fopr_off;   // HEX sides: face-modal rows BF [Nq][n^2] (-1: none)
     const int64_t* foprt_off;  // and their transpose BFT [n^2][Nq]
     const int32_t* fopr_n;     // columns of the face-modal rows (HEX, WEDGE quad: n^2; WEDGE TRI: Nt)
+    const int32_t* ftc;        // quad face-modal rows in tensor form: m | symmetry << 8 | n << 12, 0: none
+    const int64_t* fte_off;    // and their 1-D factor E [m][n] in fop
+    int tensor_faces;          // 1: the quad face maps run in tensor form where they have one
     const double* fop;
     const int64_t* fw_off;
     const double* fw;
@@ -145,6 +148,51 @@ struct MixArgs {
 // --------------------------------------------------------------------------------------------------
 // device code
 // --------------------------------------------------------------------------------------------------
+
+// The tensor form of a quad face's face-modal rows, BF[q][a + n b] = E[ia][a] E[ib][b]: face node
+// q = qx + m qy (the face rule's order, as listed by the face's left volume) sits at (ia, ib) of the side's
+// m x m grid under one of the 8 symmetries of the square, code = swap | flip_a << 1 | flip_b << 2:
+// (u, v) = swap ? (qy, qx) : (qx, qy), ia = flip_a ? m-1-u : u, ib = flip_b ? m-1-v : v.
+__device__ __forceinline__ void face_grid_of(int q, int m, int code, int& ia, int& ib)
+{
+    const int qx = q % m, qy = q / m;
+    const int u = (code & 1) ? qy : qx, v = (code & 1) ? qx : qy;
+    ia = (code & 2) ? m - 1 - u : u;
+    ib = (code & 4) ? m - 1 - v : v;
+}
+
+// The face node at (ia, ib) of the side's grid (the inverse of face_grid_of).
+__device__ __forceinline__ int face_node_of(int ia, int ib, int m, int code)
+{
+    const int u = (code & 2) ? m - 1 - ia : ia, v = (code & 4) ? m - 1 - ib : ib;
+    return (code & 1) ? v + m * u : u + m * v;
+}
+
+// A face node's value of the side's face-modal expansion (S: [n^2] coefficients, a fastest) through the
+// tensor form: sum_b E[ib][b] sum_a E[ia][a] S[a + n b], the 2n factor entries loaded once.
+template <int N>
+__device__ __forceinline__ void face_trace_tensor(const double* E, int ia, int ib, const double* S, int sv,
+                                                  double w[5])
+{
+    double ea[N], eb[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        ea[i] = E[ia * N + i];
+        eb[i] = E[ib * N + i];
+    }
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+        double t[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                t[v] = fma(ea[i], S[v * sv + i + N * b], t[v]);
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+            w[v] = fma(eb[b], t[v], w[v]);
+    }
+}
 
 template <int FLUX>
 __device__ __attribute__((always_inline)) inline void face_tile(const MixArgs& a, const FaceTile& t)
@@ -177,23 +225,88 @@ __device__ __attribute__((always_inline)) inline void face_tile(const MixArgs& a
     __syncthreads();
     // the lift of each HEX / WEDGE side in face-modal form, -+sum_q BF[q][x] nnf[q][v] (the left side takes
     // -nnf): computed here, where the face's fluxes sit in LDS, the volume kernel reads 5 x width values
-    // per face instead of a vector load per face node and row
+    // per face instead of a vector load per face node and row.  Sides whose rows have the tensor form,
+    // BF[q][a + n b] = E[ia][a] E[ib][b], contract one direction at a time through LDS:
+    //   R[v][a][ib] = sum_ia E[ia][a] nnf(node at (ia, ib)),  c[v][a + n b] = sum_ib E[ib][b] R[v][a][ib]
+    // (5 n m (n + m) multiply-adds per side instead of 5 n^2 m^2).
+    __shared__ double s_R[kBlock * 10];    // per face slot 10 tpf doubles: [side][v][a][ib], 5 n m <= 5 tpf each
+    int64_t offL = -1, offR = -1;
+    int oL = 0, oR = 0, wL = 0, wR = 0, tcL = 0, tcR = 0;
+    const double* nfs = s_nf + 5 * (slot << lg);
+    double* Rs = s_R + 10 * (slot << lg);
     if (face) {
-        const int64_t offL = a.fl_off[2 * f], offR = a.fl_off[2 * f + 1];
-        const int oL = a.face_op[2 * f], oR = a.face_op[2 * f + 1];
-        const int wL = offL >= 0 ? a.fopr_n[oL] : 0, wR = offR >= 0 ? a.fopr_n[oR] : 0;
-        const double* nfs = s_nf + 5 * (slot << lg);
+        offL = a.fl_off[2 * f];
+        offR = a.fl_off[2 * f + 1];
+        oL = a.face_op[2 * f];
+        oR = a.face_op[2 * f + 1];
+        wL = offL >= 0 ? a.fopr_n[oL] : 0;
+        wR = offR >= 0 ? a.fopr_n[oR] : 0;
+        tcL = (a.tensor_faces && offL >= 0) ? a.ftc[oL] : 0;
+        tcR = (a.tensor_faces && offR >= 0) ? a.ftc[oR] : 0;
+        // first direction: items (a, ib) of each tensor side
+        const int r1L = tcL ? (tcL >> 12) * (tcL & 255) : 0, r1R = tcR ? (tcR >> 12) * (tcR & 255) : 0;
+        for (int it = l; it < r1L + r1R; it += tpf) {
+            const int side = it >= r1L ? 1 : 0;
+            const int tc = side ? tcR : tcL;
+            const int m = tc & 255, code = (tc >> 8) & 7, n = tc >> 12;
+            const int y = it - side * r1L, xa = y % n, ib = y / n;
+            const double* E = a.fop + a.fte_off[side ? oR : oL];
+            double ea[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                ea[i] = i < m ? E[i * n + xa] : 0.0;
+            double t[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+            for (int ia = 0; ia < 8; ++ia) {
+                if (ia < m) {
+                    const double* nv = nfs + 5 * face_node_of(ia, ib, m, code);
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        t[v] = fma(ea[ia], nv[v], t[v]);
+                }
+            }
+            double* R = Rs + (side ? 5 * r1L : 0) + y;
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                R[v * n * m] = t[v];
+        }
+    }
+    __syncthreads();
+    if (face) {
+        const int r1L = tcL ? (tcL >> 12) * (tcL & 255) : 0;
         for (int it = l; it < wL + wR; it += tpf) {
             const int side = it >= wL ? 1 : 0;
             const int x = it - side * wL, wdt = side ? wR : wL;
-            const double* bf = a.fop + a.fopr_off[side ? oR : oL];
+            const int o = side ? oR : oL;
+            const int tc = side ? tcR : tcL;
             double acc[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-#pragma unroll 6
-            for (int q = 0; q < nq; ++q) {
-                const double b = bf[q * wdt + x];
+            if (tc) {
+                // second direction: c[v][a + n b] = sum_ib E[ib][b] R[v][a][ib]
+                const int m = tc & 255, n = tc >> 12;
+                const int xa = x % n, xb = x / n;
+                const double* E = a.fop + a.fte_off[o];
+                const double* R = Rs + (side ? 5 * r1L : 0) + xa;
+                double eb[8];
 #pragma unroll
-                for (int v = 0; v < 5; ++v)
-                    acc[v] = fma(b, nfs[5 * q + v], acc[v]);
+                for (int i = 0; i < 8; ++i)
+                    eb[i] = i < m ? E[i * n + xb] : 0.0;
+#pragma unroll
+                for (int ib = 0; ib < 8; ++ib) {
+                    if (ib < m) {
+#pragma unroll
+                        for (int v = 0; v < 5; ++v)
+                            acc[v] = fma(eb[ib], R[v * n * m + n * ib], acc[v]);
+                    }
+                }
+            } else {
+                const double* bf = a.fop + a.fopr_off[o];
+#pragma unroll 6
+                for (int q = 0; q < nq; ++q) {
+                    const double b = bf[q * wdt + x];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        acc[v] = fma(b, nfs[5 * q + v], acc[v]);
+                }
             }
             double* out = a.fl + (side ? offR : offL) + x;
 #pragma unroll
@@ -662,14 +775,21 @@ __device__ __attribute__((always_inline)) inline void vol_tile_hex(const MixArgs
                 const int nq = nqk[k];
                 const int f = ent >> 1, side = ent & 1;
                 const int o = a.face_op[2 * f + side];
-                const double* bt = a.fop + a.foprt_off[o];
+                const int tc = a.tensor_faces ? a.ftc[o] : 0;
                 double w[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+                if (tc) {
+                    int ia, ib;
+                    face_grid_of(q, tc & 255, (tc >> 8) & 7, ia, ib);
+                    face_trace_tensor<n>(a.fop + a.fte_off[o], ia, ib, S + k * 5 * n2, n2, w);
+                } else {
+                    const double* bt = a.fop + a.foprt_off[o];
 #pragma unroll 6
-                for (int ab = 0; ab < n2; ++ab) {
-                    const double b = bt[ab * nq + q];
+                    for (int ab = 0; ab < n2; ++ab) {
+                        const double b = bt[ab * nq + q];
 #pragma unroll
-                    for (int v = 0; v < 5; ++v)
-                        w[v] = fma(b, S[(k * 5 + v) * n2 + ab], w[v]);
+                        for (int v = 0; v < 5; ++v)
+                            w[v] = fma(b, S[(k * 5 + v) * n2 + ab], w[v]);
+                    }
                 }
                 double* t = a.tr + 2 * a.fb_off[f] + (int64_t)side * 5 * nq + 5 * q;
 #pragma unroll
@@ -944,14 +1064,22 @@ __device__ __attribute__((always_inline)) inline void vol_tile_wedge(const MixAr
                 const int f = ent >> 1, side = ent & 1;
                 const int o = a.face_op[2 * f + side];
                 const int wdt = a.fopr_n[o];
-                const double* bt = a.fop + a.foprt_off[o];
+                const int tc = a.tensor_faces ? a.ftc[o] : 0;
                 double w[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+                if (tc) {
+                    // a quad face: its coefficients c[a + n k] in tensor form
+                    int ia, ib;
+                    face_grid_of(q, tc & 255, (tc >> 8) & 7, ia, ib);
+                    face_trace_tensor<n>(a.fop + a.fte_off[o], ia, ib, S + k * 5 * n2, n2, w);
+                } else {
+                    const double* bt = a.fop + a.foprt_off[o];
 #pragma unroll 6
-                for (int x = 0; x < wdt; ++x) {
-                    const double b = bt[x * nq + q];
+                    for (int x = 0; x < wdt; ++x) {
+                        const double b = bt[x * nq + q];
 #pragma unroll
-                    for (int v = 0; v < 5; ++v)
-                        w[v] = fma(b, S[(k * 5 + v) * n2 + x], w[v]);
+                        for (int v = 0; v < 5; ++v)
+                            w[v] = fma(b, S[(k * 5 + v) * n2 + x], w[v]);
+                    }
                 }
                 double* t = a.tr + 2 * a.fb_off[f] + (int64_t)side * 5 * nq + 5 * q;
 #pragma unroll
@@ -1471,8 +1599,10 @@ struct dgx_mixed_ctx {
     int64_t *fl_off = nullptr;
     double* fl = nullptr;
     int64_t *dof = nullptr, *fb_off = nullptr, *fop_off = nullptr, *fopt_off = nullptr, *fw_off = nullptr,
-            *fopr_off = nullptr, *foprt_off = nullptr;
-    int32_t *fopr_n = nullptr;
+            *fopr_off = nullptr, *foprt_off = nullptr, *fte_off = nullptr;
+    int32_t *fopr_n = nullptr, *ftc = nullptr;
+    int tensor_faces = 1;                // quad face maps in tensor form where they factor
+    int n_tensor_ops = 0;                // face operators that have the tensor form
     int32_t *ef = nullptr, *elf = nullptr, *face_elem = nullptr, *face_op = nullptr, *face_wt = nullptr, *fop_nq = nullptr,
             *fop_ns = nullptr, *face_order = nullptr, *vorder = nullptr, *gorder = nullptr;
     GroupDev* d_groups = nullptr;
@@ -1645,6 +1775,9 @@ MixArgs make_args(dgx_mixed_ctx* c, const double* u, const double* reg, double* 
     a.fopr_off = c->fopr_off;
     a.foprt_off = c->foprt_off;
     a.fopr_n = c->fopr_n;
+    a.ftc = c->ftc;
+    a.fte_off = c->fte_off;
+    a.tensor_faces = c->tensor_faces;
     a.face_lf_of = c->elf;
     a.fop = c->fop;
     a.fw_off = c->fw_off;
@@ -1818,6 +1951,85 @@ void free_all(dgx_mixed_ctx* c)
     c->stream = nullptr;
 }
 
+// Whether a quad face's face-modal rows BF [nq][n^2] (a fastest) are a permuted tensor product,
+// BF[q][a + n b] = E[ia][a] E[ib][b] with E [m][n] the 1-D basis at m points per direction and (ia, ib)
+// the node q of the face rule's m x m grid under one of the square's 8 symmetries (face_grid_of).  The
+// points come from the ratio of each row's linear to its constant mode along a and along b; E from the
+// rows' b = 0 entries, with E[.][0] the (constant) value of the constant mode; every entry is checked
+// to 1e-12 of the table's scale.  Returns the symmetry code and fills m and E, or returns -1.
+int tensor_face_rows(const double* BF, int nq, int n, int* m_out, std::vector<double>& E)
+{
+    int m = 1;
+    while (m * m < nq)
+        ++m;
+    if (m * m != nq || n < 2 || m < n || m > 8)
+        return -1;
+    const int n2 = n * n;
+    double scale = 0.0;
+    for (int64_t i = 0; i < (int64_t)nq * n2; ++i)
+        scale = std::max(scale, std::fabs(BF[i]));
+    std::vector<double> xa(nq), xb(nq);
+    for (int q = 0; q < nq; ++q) {
+        const double c0 = BF[(int64_t)q * n2];
+        if (!(c0 > 0.0) || !std::isfinite(c0))
+            return -1;
+        xa[q] = BF[(int64_t)q * n2 + 1] / c0;
+        xb[q] = BF[(int64_t)q * n2 + n] / c0;
+    }
+    std::vector<double> srt(xa);
+    std::sort(srt.begin(), srt.end());
+    std::vector<double> pts;
+    for (double x : srt)
+        if (pts.empty() || x - pts.back() > 1e-9)
+            pts.push_back(x);
+    if ((int)pts.size() != m)
+        return -1;
+    auto index_of = [&](double x) {
+        for (int i = 0; i < m; ++i)
+            if (std::fabs(x - pts[i]) <= 1e-9)
+                return i;
+        return -1;
+    };
+    std::vector<int> ia(nq), ib(nq);
+    for (int q = 0; q < nq; ++q) {
+        ia[q] = index_of(xa[q]);
+        ib[q] = index_of(xb[q]);
+        if (ia[q] < 0 || ib[q] < 0)
+            return -1;
+    }
+    int code = -1;
+    for (int cd = 0; cd < 8 && code < 0; ++cd) {
+        bool ok = true;
+        for (int q = 0; q < nq && ok; ++q) {
+            const int qx = q % m, qy = q / m;
+            const int u = (cd & 1) ? qy : qx, v = (cd & 1) ? qx : qy;
+            ok = ia[q] == ((cd & 2) ? m - 1 - u : u) && ib[q] == ((cd & 4) ? m - 1 - v : v);
+        }
+        if (ok)
+            code = cd;
+    }
+    if (code < 0)
+        return -1;
+    E.assign((size_t)m * n, 0.0);
+    std::vector<char> have(m, 0);
+    for (int q = 0; q < nq; ++q) {
+        if (have[ia[q]])
+            continue;
+        const double e0 = std::sqrt(BF[(int64_t)q * n2]);
+        for (int a = 0; a < n; ++a)
+            E[(size_t)ia[q] * n + a] = BF[(int64_t)q * n2 + a] / e0;
+        have[ia[q]] = 1;
+    }
+    for (int q = 0; q < nq; ++q)
+        for (int b = 0; b < n; ++b)
+            for (int a = 0; a < n; ++a)
+                if (!(std::fabs(BF[(int64_t)q * n2 + a + n * b] - E[(size_t)ia[q] * n + a] * E[(size_t)ib[q] * n + b]) <=
+                      1e-12 * scale))
+                    return -1;
+    *m_out = m;
+    return code;
+}
+
 template <class T>
 dgx_status upload(dgx_mixed_ctx* c, T** dst, const std::vector<T>& h)
 {
@@ -1940,6 +2152,27 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
                     fop.push_back(m->fop_data[froff[o] + (int64_t)q * fn2[o] + j]);
         }
     }
+    // quad face-modal rows in tensor form (HEX faces, WEDGE quad faces): their 1-D factor after the data
+    std::vector<int32_t> ftc(m->n_face_op, 0);
+    std::vector<int64_t> fte(m->n_face_op, -1);
+    for (int o = 0; o < m->n_face_op; ++o) {
+        if (froff[o] < 0)
+            continue;
+        int n = 1;
+        while (n * n < fn2[o])
+            ++n;
+        if (n * n != fn2[o])
+            continue;
+        int mq = 0;
+        std::vector<double> E;
+        const int code = tensor_face_rows(m->fop_data + froff[o], fnq[o], n, &mq, E);
+        if (code < 0)
+            continue;
+        ftc[o] = mq | code << 8 | n << 12;
+        ++c->n_tensor_ops;
+        fte[o] = (int64_t)fop.size();
+        fop.insert(fop.end(), E.begin(), E.end());
+    }
     int64_t fw_len = 0;
     for (int w = 0; w < m->n_face_wt; ++w) {
         MIX_CHECK(c, m->fw_n[w] > 0 && m->fw_n[w] <= 64 && m->fw_off[w] >= 0, "face weight table");
@@ -2049,6 +2282,8 @@ dgx_status setup(dgx_mixed_ctx* c, const dgx_mixed_mesh* m)
     MIX_RUN(upload(c, &c->fopr_off, froff));
     MIX_RUN(upload(c, &c->foprt_off, frofft));
     MIX_RUN(upload(c, &c->fopr_n, fn2));
+    MIX_RUN(upload(c, &c->ftc, ftc));
+    MIX_RUN(upload(c, &c->fte_off, fte));
     MIX_RUN(upload(c, &c->elf, elf));
     MIX_RUN(upload(c, &c->fop, fop));
     MIX_RUN(upload(c, &c->fw_off, std::vector<int64_t>(m->fw_off, m->fw_off + m->n_face_wt)));
@@ -2244,6 +2479,22 @@ dgx_status dgx_mixed_get_launch(const dgx_mixed_ctx* ctx, int* mode, int32_t* ti
         for (int g = 0; g < ctx->n_group; ++g)
             tile_elems[g] = ctx->groups[g].tile;
     return DGX_OK;
+}
+
+dgx_status dgx_mixed_set_face_maps(dgx_mixed_ctx* ctx, int tensor)
+{
+    if (!ctx)
+        return DGX_ERR_INVALID_ARG;
+    MIX_CHECK(ctx, tensor == 0 || tensor == 1, "face map form");
+    MIX_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    ctx->tensor_faces = tensor;
+    return DGX_OK;
+}
+
+int32_t dgx_mixed_tensor_face_ops(const dgx_mixed_ctx* ctx)
+{
+    return ctx ? (int32_t)ctx->n_tensor_ops : -1;
 }
 
 dgx_status dgx_mixed_autotune(dgx_mixed_ctx* ctx, int reps, double* timings)

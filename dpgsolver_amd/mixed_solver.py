@@ -233,6 +233,15 @@ class MixedSolver:
         self._check(self._lib.dgx_mixed_set_launch(self._ctx, m, None if t is None else _ffi.i32ptr(t),
                                                    int(bpc if blocks_per_cu is None else blocks_per_cu)))
 
+    def set_face_maps(self, tensor=True):
+        """Quad face maps of the HEX / WEDGE sides as tensor products (True) or as the dense face-modal
+        rows (False); both give the same residual to round-off."""
+        self._check(self._lib.dgx_mixed_set_face_maps(self._ctx, int(bool(tensor))))
+
+    def tensor_face_ops(self):
+        """Number of face operators the library applies in tensor form."""
+        return int(self._lib.dgx_mixed_tensor_face_ops(self._ctx))
+
     def get_launch(self):
         mode = ctypes.c_int()
         tiles = np.empty(self.tab.n_group, dtype=np.int32)

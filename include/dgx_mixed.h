@@ -127,6 +127,17 @@ dgx_status dgx_mixed_step(dgx_mixed_ctx* ctx, double dt, int nsteps);
 dgx_status dgx_mixed_set_launch(dgx_mixed_ctx* ctx, int mode, const int32_t* tile_elems, int32_t blocks_per_cu);
 dgx_status dgx_mixed_get_launch(const dgx_mixed_ctx* ctx, int* mode, int32_t* tile_elems, int32_t* blocks_per_cu);
 
+/* Form of the quad face maps of sum-factorized HEX and WEDGE sides (the new state's traces at the face
+ * nodes, and the face flux's projection onto the side's face-modal rows for the lift): 1 (default) applies
+ * them as a tensor product, BF[q][a + n b] = E[ia][a] E[ib][b] with the 1-D factor E at the face's m points
+ * per direction and (ia, ib) the node's place in the side's grid under a symmetry of the square, wherever
+ * dgx_mixed_create found the rows to factor that way (checked to 1e-12 of the table); 0 applies every
+ * row as given.  No counterpart in the reference (its face operators are applied as dense matrices,
+ * compute_face_rlhs_T.c); both forms give the same residual to round-off.  Drops the captured graph. */
+dgx_status dgx_mixed_set_face_maps(dgx_mixed_ctx* ctx, int tensor);
+/* Face operators that have the tensor form (-1 for a NULL context). */
+int32_t dgx_mixed_tensor_face_ops(const dgx_mixed_ctx* ctx);
+
 /* The autotuner (both launch forms; the current form is kept): for every group, times the volume kernel
  * of that group alone at each legal tile_elems (reps launches each, HIP events on the context's stream)
  * and keeps the fastest for the per-group form; for the persistent form, whose one launch has one LDS

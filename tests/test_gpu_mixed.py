@@ -8,7 +8,9 @@ state:
   picks give bit-identical results (each lane's sums do not depend on the launch shape);
 - repeated graph replays equal single steps;
 - the sum-factorized volume paths (HEX and WEDGE tensor products with face-modal HEX lifts and traces,
-  TET and PYR collapsed coordinates) equal the dense path (and the oracle) to 1e-12.
+  TET and PYR collapsed coordinates) equal the dense path (and the oracle) to 1e-12;
+- the quad face maps in tensor form (every HEX face and WEDGE quad face operator) equal the dense
+  face-modal rows (and the oracle) to 1e-12.
 """
 import numpy as np
 import pytest
@@ -121,3 +123,32 @@ def test_sum_factorized_vs_dense():
     assert rel_err(a.get_state(), b.get_state()) < TOL
     a.close()
     b.close()
+
+
+def test_tensor_face_maps_vs_dense_rows():
+    _require_gpu()
+    print("case: tensor-form quad face maps vs dense face-modal rows", flush=True)
+    mesh = MM.build_mixed_mesh((5, 4, 4), seed=13)
+    u = _state(mesh, 13)
+    tab = build_tables(mesh)
+    s = MixedSolver(mesh, num_flux="roe", tables=tab)
+    # every face operator with square face-modal rows (HEX faces, WEDGE quad faces) factors
+    widths = [int(w) for off, w in zip(tab.arrays["fop_red_off"], tab.arrays["fop_red_n"]) if off >= 0]
+    n_quad = sum(1 for w in widths if int(round(w ** 0.5)) ** 2 == w)
+    assert n_quad > 0 and s.tensor_face_ops() == n_quad, (s.tensor_face_ops(), n_quad)
+    orc = MixedOracle(mesh, num_flux=1)
+    dt = MM.dt_cfl(mesh)
+    res = {}
+    for tensor in (True, False):
+        s.set_face_maps(tensor)
+        s.set_state(u)
+        r = [s.rhs(apply_minv=m) for m in (False, True)]
+        s.step(dt, 2)
+        res[tensor] = (r, s.get_state())
+    for m in range(2):
+        assert rel_err(res[True][0][m], res[False][0][m]) < TOL
+        assert rel_err(res[True][0][m], orc.residual(u, apply_minv=bool(m))) < TOL
+    assert rel_err(res[True][1], res[False][1]) < TOL
+    # the two forms sum in different orders: equal results would mean the switch changed nothing
+    assert not np.array_equal(res[True][0][0], res[False][0][0])
+    s.close()
