@@ -215,10 +215,14 @@ __device__ __attribute__((always_inline)) inline void face_tile(const MixArgs& a
         double nf[5];
         dgx::numflux<3, FLUX>(wl, wr, n, nf);
         const double sc = a.fw[a.fw_off[a.face_wt[f]] + l] * a.face_jac[f];
+        // the flux buffer is read only by the sides lifted from the face nodes (the dense lifts): a face
+        // whose two sides both lift in face-modal form does not write it
+        const bool to_fb = a.fl_off[2 * f] < 0 || a.fl_off[2 * f + 1] < 0;
         double* o = a.fb + a.fb_off[f] + 5 * l;
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
-            o[v] = nf[v] * sc;
+            if (to_fb)
+                o[v] = nf[v] * sc;
             s_nf[5 * threadIdx.x + v] = nf[v] * sc;
         }
     }
