@@ -27,6 +27,42 @@ namespace {
 
 constexpr int TG = 32;
 
+// Exchange partners of a 64-bit key within each 32-lane half of the wavefront, as two 32-bit halves:
+// DPP quad permutes (xor 1, 2), swizzles (xor 4, 8) and the gfx950 row swap (xor 16).  No LDS round trip,
+// unlike a shuffle through ds_bpermute.
+template <int CTRL>
+__device__ __forceinline__ uint64_t key_dpp(uint64_t k)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)k, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(k >> 32), CTRL, 0xF, 0xF, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <int PATTERN>
+__device__ __forceinline__ uint64_t key_swz(uint64_t k)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)k, PATTERN);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)(k >> 32), PATTERN);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// the maximum of the key over the lane's 32-lane half
+__device__ __forceinline__ uint64_t key_max32(uint64_t k)
+{
+    uint64_t o = key_dpp<0xB1>(k);      // xor 1
+    k = o > k ? o : k;
+    o = key_dpp<0x4E>(k);               // xor 2
+    k = o > k ? o : k;
+    o = key_swz<0x101F>(k);             // xor 4
+    k = o > k ? o : k;
+    o = key_swz<0x201F>(k);             // xor 8
+    k = o > k ? o : k;
+    const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)k, (uint32_t)k, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(k >> 32), (uint32_t)(k >> 32), false, false);
+    const uint64_t a = ((uint64_t)hi[0] << 32) | lo[0], b = ((uint64_t)hi[1] << 32) | lo[1];   // xor 16
+    return a > b ? a : b;
+}
+
 // The 32 (or, for the last tile, n - 32 CK) pivot steps of column tile CK and the update of every other
 // column; CK is a template argument so that every index into the register tile is a constant.
 struct InvShared {
@@ -65,11 +101,7 @@ __device__ __forceinline__ void panel(float (&a)[R][R], unsigned& used, int& fla
         const float cv = s_cval[buf][threadIdx.x & 31];
         uint64_t key = ((uint64_t)(cv > 0.f ? __float_as_uint(cv) : 0u) << 32) |
                        (uint32_t)~(uint32_t)s_cidx[buf][threadIdx.x & 31];
-#pragma unroll
-        for (int off = 16; off >= 1; off >>= 1) {
-            const uint64_t k2 = __shfl_xor(key, off);
-            key = k2 > key ? k2 : key;
-        }
+        key = key_max32(key);
         // every lane holds the same maximum: read it as a wavefront-uniform (scalar) value, so the row
         // tests below are scalar compares
         const uint32_t klo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)key);
