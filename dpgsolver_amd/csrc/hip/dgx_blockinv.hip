@@ -142,16 +142,21 @@ __device__ __forceinline__ void panel(float (&a)[R][R], unsigned& used, int& fla
         inv = (v > 0.f) ? inv : 0.f;
         const float pj = s_prow[buf][tj];
         const bool pcol_lane = tj == kk;
+        // rows other than the pivot row: a_ij - (a_ik / a_pk) a_pj, and -a_ik / a_pk in the pivot column
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = ti + TG * r;
-            const bool is_p = prow_lane && r == rp;
             const float fo = sm.pcol[buf * NP + i] * inv;
-            // pivot row: a_pj (1 - (1 - inv)) = a_pj inv; other rows: a_ij - (a_ik / a_pk) a_pj
-            const float f = is_p ? 1.f - inv : fo;
-            a[r][CK] = fmaf(-f, pj, a[r][CK]);
-            if (pcol_lane)
-                a[r][CK] = is_p ? inv : -fo;
+            const float t = fmaf(-fo, pj, a[r][CK]);
+            a[r][CK] = pcol_lane ? -fo : t;
+        }
+        if (prow_lane) {
+            // the pivot row (its element here is a_pj itself): a_pj (1 - (1 - inv)) = a_pj inv, and inv in
+            // the pivot column; only the half-wavefront holding row p runs this
+            const float pv = pcol_lane ? inv : fmaf(-(1.f - inv), pj, pj);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                a[r][CK] = (r == rp) ? pv : a[r][CK];
         }
 
         // pivot candidates of column k + 1 within the panel (unused rows only)
