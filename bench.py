@@ -303,17 +303,19 @@ def ke_history(dg, torch, solver, u0, dt, steps, every):
 
 def ns_small_vs_oracle(dg, steps):
     """The Navier-Stokes protocol (TGV Re=1600, M=0.1, HEX p=4, Roe + BR2, SSP-RK3) at the size the host
-    oracle finishes in about a minute (6^3): state and kinetic-energy error after `steps` steps, next to the
-    GPU round-off floor of the state (the same steps from a 1e-16 relative perturbation of the start)."""
+    oracle (sum-factorized form, all host threads) finishes in about a minute (10^3): state and
+    kinetic-energy error after `steps` steps, next to the GPU round-off floor of the state (the same steps
+    from a 1e-16 relative perturbation of the start)."""
     import numpy as np
     from oracle.oracle import Oracle
-    n, p, visc = 6, 4, (1.0 / 1600.0, 0.71)
+    n, p, visc = 10, 4, (1.0 / 1600.0, 0.71)
     mesh = dg.structured_mesh(3, (n, n, n), (0, 0, 0), (2 * math.pi,) * 3, p_g=1)
     geo = dg.geometry(mesh, p)
     u0 = dg.ic_taylor_green(mesh, p, 0.1)
     dt = dg.dt_cfl(mesh, p, u0, 0.5)
     orc = Oracle(mesh, p, num_flux=1, geometry=geo, viscous=visc)
     orc.set_threads(host_threads())
+    orc.set_sum_factorized(True)
     u, up = u0.copy(), np.zeros_like(u0)
     orc.step(dt, u, up, steps)
     s = dg.Solver(mesh, geo, p, num_flux="roe", viscous=visc)
